@@ -1,0 +1,87 @@
+/*
+ * bp_dev.h -- the device-resident graph shared by the loopy (bp_device.hip)
+ * and tree (bp_tree.hip) translation units of libbp_amd.so.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "bp_internal.h"
+#include "bp_kernels.h"
+#include "../../include/bp_hip.h"
+
+extern "C" void bp_set_error(const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t err_ = (expr);                                                            \
+        if (err_ != hipSuccess) {                                                            \
+            bp_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(err_), __FILE__, \
+                         __LINE__);                                                          \
+            return -1;                                                                       \
+        }                                                                                    \
+    } while (0)
+
+struct bp_tree_plan;
+
+struct bp_dev {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    unsigned V = 0, E = 0, S = 0, SP = 0;
+    bool mixed = false;
+
+    unsigned *in_off = nullptr;       /* [V+1] slot range per node */
+    unsigned *slot_src = nullptr;     /* [E] */
+    unsigned char *slot_xy = nullptr; /* [E] */
+    float *J = nullptr;               /* [E][S*S] */
+    unsigned char *nvars = nullptr;   /* [V] */
+    float *states = nullptr;          /* [V][SP] */
+    float *buf[2] = {nullptr, nullptr};
+    float *msg[2] = {nullptr, nullptr};
+    float *tree_msg = nullptr;        /* [E][SP] the reference's edges_messages, slot order */
+    unsigned *chunk = nullptr;
+    unsigned nchunks = 0, G = 0, cpb = 0;
+
+    double *partials = nullptr; /* [PART_CAP][G] */
+    double *deltas = nullptr;   /* [PART_CAP] */
+    double *last = nullptr;
+    unsigned *stop = nullptr;
+
+    /* host mirrors for upload / download */
+    std::vector<unsigned> slot_edge; /* slot -> reference edge index */
+    std::vector<unsigned> edge_slot; /* reference edge index -> slot */
+    std::vector<unsigned> node_n;    /* state count per node */
+    std::vector<unsigned> edge_x;    /* x per reference edge */
+    double alg_bytes = 0.0;
+
+    /* run state */
+    int latest = 0;       /* msg[latest] holds the newest messages */
+    int bcur = 0;         /* buf[bcur] holds the combine buffer of the next iteration */
+    bool buf_valid = false;
+    unsigned swaps = 0;   /* iterations since upload: host pointer swaps owed */
+    bool tree_valid = false;
+
+    std::map<unsigned long long, hipGraphExec_t> graphs;
+
+    bp_tree_plan *tree = nullptr;
+};
+
+inline unsigned stride_of(unsigned S) { return S <= 2 ? S : S <= 4 ? 4 : S <= 8 ? 8 : 10; }
+
+template <typename T> static int dmalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0)
+        count = 1;
+    HIP_TRY(hipMalloc((void **)p, count * sizeof(T)));
+    return 0;
+}
+
+/* tree BP (bp_tree.hip) */
+int bp_tree_plan_build(bp_dev *d, Graph_t g);
+void bp_tree_plan_free(bp_tree_plan *p);
+unsigned long long bp_tree_plan_waves(const bp_tree_plan *p);
+int bp_download_tree_msgs(bp_dev *d, Graph_t g);

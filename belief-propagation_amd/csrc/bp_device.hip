@@ -1,0 +1,852 @@
+/*
+ * bp_device.hip -- device-resident graph, loopy BP launches, the C-ABI of
+ * include/bp_hip.h and the GPU-executed propagation entry points of the
+ * drop-in graph API (include/graph.h).
+ *
+ * Replaces the host orchestration of src/cuda_benchmark/belief-propagation.cu
+ * (:529-661 node variant, :664-783 edge variant): one upload of a
+ * destination-grouped structure-of-arrays graph, one kernel per loopy
+ * iteration replayed from a cached HIP graph, the FP64 delta reduced once per
+ * batch (fixed-iteration runs) or after every iteration with an on-device
+ * stop flag (convergence runs), and one download.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "bp_dev.h"
+#include "../../include/bp_hip.h"
+
+using namespace bpk;
+
+/* Sums the per-block partials of `count` launches (fixed order, so the
+ * result is reproducible).  until != 0: evaluates the stop test of
+ * graph.c:1313 for launch `index` against *last and raises *stop. */
+__global__ __launch_bounds__(BLOCK) void k_finish(const double *partials, unsigned G, unsigned count,
+                                                 double *deltas, double *last, unsigned *stop, float conv,
+                                                 int until, unsigned index)
+{
+    __shared__ double red[BLOCK];
+    if (until && *stop)
+        return;
+    for (unsigned it = 0; it < count; ++it) {
+        const double *p = partials + (size_t)it * G;
+        double acc = 0.0;
+        for (unsigned i = threadIdx.x; i < G; i += BLOCK)
+            acc += p[i];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (unsigned w = BLOCK / 2; w > 0; w >>= 1) {
+            if (threadIdx.x < w)
+                red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            double d = red[0];
+            deltas[until ? index : it] = d;
+            if (until) {
+                double prev = *last;
+                double change = d - prev;
+                if (change < 0.0)
+                    change = -change;
+                if (d < (double)conv || change < (double)conv)
+                    *stop = index + 1;
+                *last = d;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_until_reset(double *last, unsigned *stop)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *last = -1.0;
+        *stop = 0;
+    }
+}
+
+/* One message on one edge (the drop-in send_message). */
+template <int S>
+__global__ void k_send_one(const float *J, const float *b, unsigned x, unsigned y, float *out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        float m[S];
+        send<S, true>(J, b, x, y, m);
+        for (unsigned i = 0; i < x; ++i)
+            out[i] = m[i];
+    }
+}
+
+
+static const unsigned PART_CAP = 256; /* iterations per delta batch */
+
+static void free_dev_arrays(bp_dev *d)
+{
+    void *ptrs[] = {d->in_off, d->slot_src, d->slot_xy, d->J, d->nvars, d->states, d->buf[0], d->buf[1],
+                    d->msg[0], d->msg[1], d->tree_msg, d->chunk, d->partials, d->deltas, d->last, d->stop};
+    for (void *p : ptrs)
+        if (p)
+            (void)hipFree(p);
+}
+
+extern "C" void bp_free(bp_dev *d)
+{
+    if (!d)
+        return;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(d->device);
+    if (d->stream)
+        (void)hipStreamSynchronize(d->stream);
+    for (auto &kv : d->graphs)
+        (void)hipGraphExecDestroy(kv.second);
+    bp_tree_plan_free(d->tree);
+    free_dev_arrays(d);
+    if (d->stream)
+        (void)hipStreamDestroy(d->stream);
+    (void)hipSetDevice(prev);
+    delete d;
+}
+
+extern "C" int bp_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        return 0;
+    return n;
+}
+
+/* Chunks: consecutive nodes with at most 64 in-slots and 64 nodes, or one
+ * node with more in-slots. */
+static void build_chunks(const std::vector<unsigned> &in_off, unsigned V, std::vector<unsigned> &chunks)
+{
+    chunks.clear();
+    unsigned v = 0;
+    while (v < V) {
+        chunks.push_back(v);
+        unsigned s0 = in_off[v];
+        unsigned w = v;
+        while (w < V && w - v < (unsigned)WAVE && in_off[w + 1] - s0 <= (unsigned)WAVE)
+            ++w;
+        if (w == v)
+            w = v + 1; /* a node with more than 64 in-slots gets a chunk of its own */
+        v = w;
+    }
+    chunks.push_back(V);
+}
+
+static int check_graph(Graph_t g)
+{
+    if (!g) {
+        bp_set_error("graph is NULL");
+        return -1;
+    }
+    if (g->current_num_vertices != g->total_num_vertices) {
+        bp_set_error("graph has %u of %u nodes", g->current_num_vertices, g->total_num_vertices);
+        return -1;
+    }
+    unsigned V = g->current_num_vertices, E = g->current_num_edges;
+    /* the destination CSR must describe the edges (set_up_dest_nodes_to_edges) */
+    for (unsigned v = 0; v < V; ++v) {
+        unsigned b = g->dest_nodes_to_edges_node_list[v], e = bp_csr_end(g, g->dest_nodes_to_edges_node_list, v);
+        if (b > e || e > E) {
+            bp_set_error("destination CSR is not set up (call set_up_dest_nodes_to_edges)");
+            return -1;
+        }
+        for (unsigned k = b; k < e; ++k) {
+            unsigned edge = g->dest_nodes_to_edges_edge_list[k];
+            if (edge >= E || g->edges_dest_index[edge] != v) {
+                bp_set_error("destination CSR is not set up (call set_up_dest_nodes_to_edges)");
+                return -1;
+            }
+        }
+    }
+    if (V > 0 && bp_csr_end(g, g->dest_nodes_to_edges_node_list, V - 1) != E) {
+        bp_set_error("destination CSR does not cover all %u edges", E);
+        return -1;
+    }
+    for (unsigned e = 0; e < E; ++e) {
+        unsigned s = g->edges_src_index[e], t = g->edges_dest_index[e];
+        if (s >= V || t >= V || g->edges_x_dim[e] != g->node_num_vars[s] || g->edges_y_dim[e] != g->node_num_vars[t]) {
+            bp_set_error("edge %u is inconsistent with its nodes", e);
+            return -1;
+        }
+    }
+    for (unsigned v = 0; v < V; ++v)
+        if (g->node_num_vars[v] > MAX_STATES) {
+            bp_set_error("node %u has %u states (limit %d)", v, g->node_num_vars[v], MAX_STATES);
+            return -1;
+        }
+    return 0;
+}
+
+/* host rows (MAX_STATES stride, first n valid) -> device rows (SP stride, zero padded) */
+static void pack_rows(const float *src, const unsigned *valid, size_t count, unsigned SP, std::vector<float> &out,
+                      const unsigned *order = nullptr)
+{
+    out.assign(count * SP, 0.0f);
+    for (size_t i = 0; i < count; ++i) {
+        size_t from = order ? order[i] : i;
+        unsigned n = valid[i];
+        for (unsigned k = 0; k < n && k < SP; ++k)
+            out[i * SP + k] = src[from * MAX_STATES + k];
+    }
+}
+
+static int upload_state(bp_dev *d, Graph_t g, bool tables)
+{
+    std::vector<float> tmp;
+    /* node states */
+    pack_rows(g->node_states, d->node_n.data(), d->V, d->SP, tmp);
+    HIP_TRY(hipMemcpyAsync(d->states, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    /* messages in slot order: msg[0] = previous, msg[1] = current */
+    std::vector<unsigned> slot_x(d->E);
+    for (unsigned s = 0; s < d->E; ++s)
+        slot_x[s] = d->edge_x[d->slot_edge[s]];
+    pack_rows(*g->previous_edge_messages, slot_x.data(), d->E, d->SP, tmp, d->slot_edge.data());
+    HIP_TRY(hipMemcpyAsync(d->msg[0], tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    pack_rows(*g->current_edge_messages, slot_x.data(), d->E, d->SP, tmp, d->slot_edge.data());
+    HIP_TRY(hipMemcpyAsync(d->msg[1], tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    pack_rows(g->edges_messages, slot_x.data(), d->E, d->SP, tmp, d->slot_edge.data());
+    HIP_TRY(hipMemcpyAsync(d->tree_msg, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    if (tables) {
+        unsigned S = d->S;
+        tmp.assign((size_t)d->E * S * S, 0.0f);
+        for (unsigned s = 0; s < d->E; ++s) {
+            unsigned e = d->slot_edge[s];
+            unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
+            const float *J = g->edges_joint_probabilities + (size_t)e * MAX_STATES * MAX_STATES;
+            for (unsigned i = 0; i < x; ++i)
+                for (unsigned j = 0; j < y; ++j)
+                    tmp[(size_t)s * S * S + i * S + j] = J[i * MAX_STATES + j];
+        }
+        HIP_TRY(hipMemcpyAsync(d->J, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+    }
+    d->latest = 0;
+    d->bcur = 0;
+    d->buf_valid = false;
+    d->swaps = 0;
+    d->tree_valid = false;
+    return 0;
+}
+
+static double algorithmic_bytes(Graph_t g)
+{
+    /* distinct tables (a transposed reverse pair counts once), per-edge and
+     * per-node traffic of one iteration */
+    double bytes = 0.0;
+    unsigned E = g->current_num_edges, V = g->current_num_vertices;
+    std::vector<unsigned char> paired(E, 0);
+    /* a reverse edge emitted by the parsers directly follows its forward edge */
+    for (unsigned e = 0; e + 1 < E; ++e) {
+        if (paired[e])
+            continue;
+        unsigned f = e + 1;
+        if (g->edges_src_index[f] == g->edges_dest_index[e] && g->edges_dest_index[f] == g->edges_src_index[e]) {
+            unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
+            const float *A = g->edges_joint_probabilities + (size_t)e * 100;
+            const float *B = g->edges_joint_probabilities + (size_t)f * 100;
+            bool t = true;
+            for (unsigned i = 0; i < x && t; ++i)
+                for (unsigned j = 0; j < y && t; ++j)
+                    t = (memcmp(&A[i * 10 + j], &B[j * 10 + i], 4) == 0);
+            if (t)
+                paired[f] = 1;
+        }
+    }
+    for (unsigned e = 0; e < E; ++e) {
+        double x = g->edges_x_dim[e], y = g->edges_y_dim[e];
+        if (!paired[e])
+            bytes += 4.0 * x * y;
+        bytes += 4.0 * (2.0 * x + y + 1.0);
+    }
+    for (unsigned v = 0; v < V; ++v)
+        bytes += 12.0 * g->node_num_vars[v] + 4.0;
+    return bytes;
+}
+
+static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
+{
+    bp_clear_error();
+    *out = nullptr;
+    if (check_graph(g))
+        return -1;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        bp_set_error("no HIP device available: the belief-propagation kernels need an MI355X (gfx950)");
+        return -1;
+    }
+    bp_dev *d = new bp_dev();
+    if (device < 0)
+        HIP_TRY(hipGetDevice(&d->device));
+    else
+        d->device = device;
+    HIP_TRY(hipSetDevice(d->device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, d->device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        bp_set_error("device %d is %s; libbp_amd.so is built for gfx950 only", d->device, prop.gcnArchName);
+        delete d;
+        return -1;
+    }
+    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+        bp_set_error("hipStreamCreate failed");
+        delete d;
+        return -1;
+    }
+
+    unsigned V = g->current_num_vertices, E = g->current_num_edges;
+    d->V = V;
+    d->E = E;
+    unsigned S = 1, smin = MAX_STATES;
+    d->node_n.assign(g->node_num_vars, g->node_num_vars + V);
+    for (unsigned v = 0; v < V; ++v) {
+        S = std::max(S, d->node_n[v]);
+        smin = std::min(smin, d->node_n[v]);
+    }
+    if (V == 0)
+        smin = S;
+    d->S = S;
+    d->SP = stride_of(S);
+    d->mixed = (smin != S);
+    d->edge_x.assign(g->edges_x_dim, g->edges_x_dim + E);
+
+    std::vector<unsigned> in_off(V + 1);
+    for (unsigned v = 0; v < V; ++v)
+        in_off[v] = g->dest_nodes_to_edges_node_list[v];
+    in_off[V] = E;
+    d->slot_edge.assign(g->dest_nodes_to_edges_edge_list, g->dest_nodes_to_edges_edge_list + E);
+    d->edge_slot.assign(E, 0);
+    for (unsigned s = 0; s < E; ++s)
+        d->edge_slot[d->slot_edge[s]] = s;
+    std::vector<unsigned> slot_src(E);
+    std::vector<unsigned char> slot_xy(E);
+    for (unsigned s = 0; s < E; ++s) {
+        unsigned e = d->slot_edge[s];
+        slot_src[s] = g->edges_src_index[e];
+        slot_xy[s] = (unsigned char)(g->edges_x_dim[e] | (g->edges_y_dim[e] << 4));
+    }
+    std::vector<unsigned> chunks;
+    build_chunks(in_off, V, chunks);
+    d->nchunks = (unsigned)chunks.size() - 1;
+    /* grid: a multiple of 8 blocks (one range per XCD), at most 2048 */
+    unsigned want = (d->nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    unsigned G = std::min(2048u, std::max(8u, (want + 7u) & ~7u));
+    d->G = G;
+    d->cpb = (d->nchunks + G - 1) / G;
+    if (d->cpb == 0)
+        d->cpb = 1;
+    d->alg_bytes = algorithmic_bytes(g);
+
+    size_t SP = d->SP;
+    if (dmalloc(&d->in_off, V + 1) || dmalloc(&d->slot_src, E) || dmalloc(&d->slot_xy, E) ||
+        dmalloc(&d->J, (size_t)E * S * S) || dmalloc(&d->nvars, V) || dmalloc(&d->states, V * SP) ||
+        dmalloc(&d->buf[0], V * SP) || dmalloc(&d->buf[1], V * SP) || dmalloc(&d->msg[0], (size_t)E * SP) ||
+        dmalloc(&d->msg[1], (size_t)E * SP) || dmalloc(&d->tree_msg, (size_t)E * SP) ||
+        dmalloc(&d->chunk, chunks.size()) || dmalloc(&d->partials, (size_t)PART_CAP * G) ||
+        dmalloc(&d->deltas, PART_CAP) || dmalloc(&d->last, 1) || dmalloc(&d->stop, 1)) {
+        bp_free(d);
+        return -1;
+    }
+    std::vector<unsigned char> nv(V);
+    for (unsigned v = 0; v < V; ++v)
+        nv[v] = (unsigned char)d->node_n[v];
+    bool ok = hipMemcpy(d->in_off, in_off.data(), (V + 1) * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d->slot_src, slot_src.data(), (size_t)E * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d->slot_xy, slot_xy.data(), E, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d->nvars, nv.data(), V, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d->chunk, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemset(d->buf[0], 0, V * SP * 4) == hipSuccess && hipMemset(d->buf[1], 0, V * SP * 4) == hipSuccess &&
+              hipMemset(d->stop, 0, 4) == hipSuccess;
+    if (!ok) {
+        bp_set_error("upload of the graph structure failed");
+        bp_free(d);
+        return -1;
+    }
+    if (upload_state(d, g, true)) {
+        bp_free(d);
+        return -1;
+    }
+    /* the tree schedule needs levels; without them bp_tree() reports it */
+    if (plan && g->num_levels > 0 && bp_tree_plan_build(d, g) != 0)
+        bp_clear_error();
+    *out = d;
+    return 0;
+}
+
+extern "C" int bp_upload(Graph_t g, int device, bp_dev **out) { return upload_impl(g, device, out, true); }
+
+extern "C" int bp_upload_state(bp_dev *d, Graph_t g)
+{
+    bp_clear_error();
+    if (!d || !g || g->current_num_vertices != d->V || g->current_num_edges != d->E) {
+        bp_set_error("bp_upload_state: graph does not match the handle");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    return upload_state(d, g, true);
+}
+
+/* ------------------------------------------------------------------------ */
+/* launches                                                                  */
+
+static LoopyArgs make_args(bp_dev *d)
+{
+    LoopyArgs a;
+    a.in_off = d->in_off;
+    a.slot_src = d->slot_src;
+    a.slot_xy = d->slot_xy;
+    a.J = d->J;
+    a.nvars = d->nvars;
+    a.states = d->states;
+    a.buf_in = d->buf[d->bcur];
+    a.buf_out = d->buf[d->bcur ^ 1];
+    a.msg_in = d->msg[d->latest];
+    a.msg_out = d->msg[d->latest ^ 1];
+    a.chunk = d->chunk;
+    a.nchunks = d->nchunks;
+    a.chunks_per_block = d->cpb;
+    a.partials = d->partials;
+    a.stop = nullptr;
+    return a;
+}
+
+template <int S, bool MIXED> static void launch_s(int mode, const LoopyArgs &a, unsigned G, hipStream_t st)
+{
+    dim3 grid(G), block(BLOCK);
+    if (mode == ITER)
+        hipLaunchKernelGGL((k_loopy<S, MIXED, ITER>), grid, block, 0, st, a);
+    else if (mode == INIT)
+        hipLaunchKernelGGL((k_loopy<S, MIXED, INIT>), grid, block, 0, st, a);
+    else
+        hipLaunchKernelGGL((k_loopy<S, MIXED, PREP>), grid, block, 0, st, a);
+}
+
+template <bool MIXED> static void launch_m(unsigned S, int mode, const LoopyArgs &a, unsigned G, hipStream_t st)
+{
+    switch (S) {
+    case 1: launch_s<1, MIXED>(mode, a, G, st); break;
+    case 2: launch_s<2, MIXED>(mode, a, G, st); break;
+    case 3: launch_s<3, MIXED>(mode, a, G, st); break;
+    case 4: launch_s<4, MIXED>(mode, a, G, st); break;
+    case 5: launch_s<5, MIXED>(mode, a, G, st); break;
+    case 6: launch_s<6, MIXED>(mode, a, G, st); break;
+    case 7: launch_s<7, MIXED>(mode, a, G, st); break;
+    case 8: launch_s<8, MIXED>(mode, a, G, st); break;
+    case 9: launch_s<9, MIXED>(mode, a, G, st); break;
+    default: launch_s<10, MIXED>(mode, a, G, st); break;
+    }
+}
+
+static void launch(bp_dev *d, int mode, const LoopyArgs &a)
+{
+    if (d->V == 0)
+        return;
+    if (d->mixed)
+        launch_m<true>(d->S, mode, a, d->G, d->stream);
+    else
+        launch_m<false>(d->S, mode, a, d->G, d->stream);
+}
+
+/* states (.) prod(previous messages) into buf[bcur] */
+static int ensure_buffer(bp_dev *d)
+{
+    if (d->buf_valid)
+        return 0;
+    LoopyArgs a = make_args(d);
+    a.buf_out = d->buf[d->bcur];
+    launch(d, PREP, a);
+    HIP_TRY(hipGetLastError());
+    d->buf_valid = true;
+    return 0;
+}
+
+extern "C" int bp_loopy_init(bp_dev *d)
+{
+    bp_clear_error();
+    if (!d) {
+        bp_set_error("bp_loopy_init: NULL handle");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    LoopyArgs a = make_args(d);
+    a.msg_out = d->msg[d->latest]; /* the previous-message array, graph.c:734 */
+    a.buf_out = d->buf[d->bcur];
+    launch(d, INIT, a);
+    HIP_TRY(hipGetLastError());
+    d->buf_valid = true;
+    return 0;
+}
+
+/* Records n iterations (+ the batch's delta reduction) on the stream. */
+static void enqueue_iterations(bp_dev *d, unsigned n, int latest, int bcur)
+{
+    for (unsigned i = 0; i < n; ++i) {
+        LoopyArgs a = make_args(d);
+        a.msg_in = d->msg[latest];
+        a.msg_out = d->msg[latest ^ 1];
+        a.buf_in = d->buf[bcur];
+        a.buf_out = d->buf[bcur ^ 1];
+        a.partials = d->partials + (size_t)i * d->G;
+        launch(d, ITER, a);
+        latest ^= 1;
+        bcur ^= 1;
+    }
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, d->stream, d->partials, d->G, n, d->deltas, d->last,
+                       d->stop, 0.0f, 0, 0u);
+}
+
+extern "C" int bp_loopy_iterate(bp_dev *d, unsigned int n, double *deltas)
+{
+    bp_clear_error();
+    if (!d) {
+        bp_set_error("bp_loopy_iterate: NULL handle");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    if (ensure_buffer(d))
+        return -1;
+    unsigned done = 0;
+    while (done < n) {
+        unsigned batch = std::min(n - done, PART_CAP);
+        unsigned long long key = ((unsigned long long)batch << 2) | ((unsigned)d->latest << 1) | (unsigned)d->bcur;
+        auto it = d->graphs.find(key);
+        if (it == d->graphs.end()) {
+            hipGraph_t graph;
+            HIP_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+            enqueue_iterations(d, batch, d->latest, d->bcur);
+            HIP_TRY(hipStreamEndCapture(d->stream, &graph));
+            hipGraphExec_t exec;
+            hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) {
+                bp_set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
+                return -1;
+            }
+            it = d->graphs.emplace(key, exec).first;
+        }
+        HIP_TRY(hipGraphLaunch(it->second, d->stream));
+        if (batch & 1u) {
+            d->latest ^= 1;
+            d->bcur ^= 1;
+        }
+        d->swaps += batch;
+        if (deltas) {
+            HIP_TRY(hipMemcpyAsync(deltas + done, d->deltas, batch * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+            HIP_TRY(hipStreamSynchronize(d->stream));
+        }
+        done += batch;
+    }
+    return 0;
+}
+
+extern "C" int bp_loopy_until(bp_dev *d, float convergence, unsigned int max_iterations, unsigned int *result)
+{
+    bp_clear_error();
+    if (!d || !result) {
+        bp_set_error("bp_loopy_until: NULL argument");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    if (ensure_buffer(d))
+        return -1;
+    hipLaunchKernelGGL(k_until_reset, dim3(1), dim3(64), 0, d->stream, d->last, d->stop);
+    HIP_TRY(hipGetLastError());
+    const unsigned batch = 16;
+    unsigned done = 0;
+    unsigned stop = 0;
+    while (done < max_iterations) {
+        unsigned n = std::min(batch, max_iterations - done);
+        int latest = d->latest, bcur = d->bcur;
+        for (unsigned i = 0; i < n; ++i) {
+            LoopyArgs a = make_args(d);
+            a.msg_in = d->msg[latest];
+            a.msg_out = d->msg[latest ^ 1];
+            a.buf_in = d->buf[bcur];
+            a.buf_out = d->buf[bcur ^ 1];
+            a.partials = d->partials;
+            a.stop = d->stop;
+            launch(d, ITER, a);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, d->stream, d->partials, d->G, 1u, d->deltas,
+                               d->last, d->stop, convergence, 1, i);
+            latest ^= 1;
+            bcur ^= 1;
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&stop, d->stop, sizeof(unsigned), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        unsigned ran = stop ? stop : n; /* launches after the stop returned at once */
+        if (ran & 1u) {
+            d->latest ^= 1;
+            d->bcur ^= 1;
+        }
+        d->swaps += ran;
+        done += ran;
+        if (stop) {
+            hipLaunchKernelGGL(k_until_reset, dim3(1), dim3(64), 0, d->stream, d->last, d->stop);
+            HIP_TRY(hipStreamSynchronize(d->stream));
+            *result = done - 1; /* the reference returns the index of the converged iteration */
+            return 0;
+        }
+        /* the next batch continues the same run: keep *last, clear nothing */
+    }
+    hipLaunchKernelGGL(k_until_reset, dim3(1), dim3(64), 0, d->stream, d->last, d->stop);
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    *result = max_iterations;
+    return 0;
+}
+
+extern "C" int bp_sync(bp_dev *d)
+{
+    bp_clear_error();
+    if (!d) {
+        bp_set_error("bp_sync: NULL handle");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+extern "C" int bp_profile_loopy(bp_dev *d, unsigned int n, float *ms)
+{
+    bp_clear_error();
+    if (!d || !ms) {
+        bp_set_error("bp_profile_loopy: NULL argument");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    if (ensure_buffer(d))
+        return -1;
+    std::vector<hipEvent_t> ev(n + 1);
+    for (auto &e : ev)
+        HIP_TRY(hipEventCreate(&e));
+    unsigned done = 0;
+    while (done < n) {
+        unsigned batch = std::min(n - done, PART_CAP);
+        for (unsigned i = 0; i < batch; ++i) {
+            LoopyArgs a = make_args(d);
+            a.partials = d->partials + (size_t)i * d->G;
+            HIP_TRY(hipEventRecord(ev[done + i], d->stream));
+            launch(d, ITER, a);
+            d->latest ^= 1;
+            d->bcur ^= 1;
+            d->swaps++;
+        }
+        HIP_TRY(hipEventRecord(ev[done + batch], d->stream));
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, d->stream, d->partials, d->G, batch, d->deltas,
+                           d->last, d->stop, 0.0f, 0, 0u);
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        for (unsigned i = 0; i < batch; ++i)
+            HIP_TRY(hipEventElapsedTime(&ms[done + i], ev[done + i], ev[done + i + 1]));
+        done += batch;
+    }
+    for (auto &e : ev)
+        (void)hipEventDestroy(e);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* download                                                                  */
+
+extern "C" int bp_download_states(bp_dev *d, float *out)
+{
+    bp_clear_error();
+    HIP_TRY(hipSetDevice(d->device));
+    HIP_TRY(hipMemcpyAsync(out, d->states, (size_t)d->V * d->SP * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+static int download_msgs(bp_dev *d, const float *dev, float *host)
+{
+    std::vector<float> tmp((size_t)d->E * d->SP);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    for (unsigned s = 0; s < d->E; ++s) {
+        unsigned e = d->slot_edge[s];
+        unsigned x = d->edge_x[e];
+        for (unsigned k = 0; k < x; ++k)
+            host[(size_t)e * MAX_STATES + k] = tmp[(size_t)s * d->SP + k];
+    }
+    return 0;
+}
+
+extern "C" int bp_download(bp_dev *d, Graph_t g)
+{
+    bp_clear_error();
+    if (!d || !g || g->current_num_vertices != d->V || g->current_num_edges != d->E) {
+        bp_set_error("bp_download: graph does not match the handle");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    std::vector<float> st((size_t)d->V * d->SP);
+    HIP_TRY(hipMemcpyAsync(st.data(), d->states, st.size() * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    for (unsigned v = 0; v < d->V; ++v)
+        for (unsigned k = 0; k < d->node_n[v]; ++k)
+            g->node_states[(size_t)v * MAX_STATES + k] = st[(size_t)v * d->SP + k];
+    /* one pointer swap per iteration run on the device (graph.c:1179-1182) */
+    if (d->swaps & 1u) {
+        float **t = g->previous_edge_messages;
+        g->previous_edge_messages = g->current_edge_messages;
+        g->current_edge_messages = t;
+    }
+    d->swaps = 0;
+    /* the device buffers keep their roles (cached HIP graphs hold their
+     * addresses); msg[latest] is the newest, i.e. the host's previous array */
+    if (download_msgs(d, d->msg[d->latest], *g->previous_edge_messages) ||
+        download_msgs(d, d->msg[d->latest ^ 1], *g->current_edge_messages))
+        return -1;
+    if (d->tree_valid && download_msgs(d, d->tree_msg, g->edges_messages))
+        return -1;
+    return 0;
+}
+
+extern "C" int bp_dev_info(const bp_dev *d, uint64_t *info)
+{
+    if (!d || !info)
+        return -1;
+    info[0] = d->V;
+    info[1] = d->E;
+    info[2] = d->SP;
+    info[3] = d->S;
+    info[4] = d->nchunks;
+    info[5] = d->mixed ? 1 : 0;
+    info[6] = bp_tree_plan_waves(d->tree);
+    return 0;
+}
+
+extern "C" double bp_loopy_algorithmic_bytes(const bp_dev *d) { return d ? d->alg_bytes : 0.0; }
+
+/* ------------------------------------------------------------------------ */
+/* drop-in entry points (include/graph.h)                                    */
+
+/* The device mirror of a Graph_t, rebuilt when its structure changed;
+ * states, tables and messages are re-read on every call because callers
+ * may write the public arrays between calls. */
+extern "C" bp_dev *bp_dropin_device(Graph_t g, const char *who)
+{
+    struct bp_graph_impl *impl = bp_impl(g);
+    bp_dev *d = (bp_dev *)impl->dev_cache;
+    if (d && impl->dev_cache_generation == impl->generation) {
+        if (bp_upload_state(d, g) == 0)
+            return d;
+        bp_fail("%s: %s", who, bp_last_error());
+        return nullptr;
+    }
+    bp_dev_cache_release(impl);
+    if (upload_impl(g, -1, &d, false) != 0) {
+        bp_fail("%s: %s", who, bp_last_error());
+        return nullptr;
+    }
+    impl->dev_cache = d;
+    impl->dev_cache_generation = impl->generation;
+    return d;
+}
+
+extern "C" void bp_dev_cache_release(struct bp_graph_impl *impl)
+{
+    if (impl && impl->dev_cache) {
+        bp_free((bp_dev *)impl->dev_cache);
+        impl->dev_cache = nullptr;
+    }
+}
+
+extern "C" void init_previous_edge(Graph_t g)
+{
+    bp_dev *d = bp_dropin_device(g, "init_previous_edge");
+    if (!d)
+        return;
+    if (bp_loopy_init(d) || bp_download(d, g))
+        bp_fail("init_previous_edge: %s", bp_last_error());
+}
+
+extern "C" void loopy_propagate_one_iteration(Graph_t g)
+{
+    bp_dev *d = bp_dropin_device(g, "loopy_propagate_one_iteration");
+    if (!d)
+        return;
+    if (bp_loopy_iterate(d, 1, nullptr) || bp_download(d, g))
+        bp_fail("loopy_propagate_one_iteration: %s", bp_last_error());
+}
+
+extern "C" unsigned int loopy_propagate_until(Graph_t g, float convergence, unsigned int max_iterations)
+{
+    bp_dev *d = bp_dropin_device(g, "loopy_propagate_until");
+    if (!d)
+        return 0;
+    unsigned int result = 0;
+    if (bp_loopy_until(d, convergence, max_iterations, &result) || bp_download(d, g)) {
+        bp_fail("loopy_propagate_until: %s", bp_last_error());
+        return 0;
+    }
+    if (result == max_iterations)
+        printf("No Convergence: after %u iterations\n", max_iterations);
+    return result;
+}
+
+extern "C" unsigned int loopy_propagate_until_hip(Graph_t g, float convergence, unsigned int max_iterations)
+{
+    bp_dev *d = nullptr;
+    bp_clear_error();
+    if (bp_upload(g, -1, &d) != 0)
+        return 0xffffffffu;
+    unsigned int result = 0;
+    int rc = bp_loopy_init(d);
+    if (rc == 0)
+        rc = bp_loopy_until(d, convergence, max_iterations, &result);
+    if (rc == 0)
+        rc = bp_download(d, g);
+    bp_free(d);
+    return rc == 0 ? result : 0xffffffffu;
+}
+
+/* graph.c:426-448 on the GPU: one edge, tables in MAX_STATES stride */
+extern "C" void send_message(float *states, unsigned int offset, unsigned int edge_index,
+                             float *edge_joint_probabilities, float *edge_messages, unsigned int *edge_num_src,
+                             unsigned int *edge_num_dest)
+{
+    unsigned x = edge_num_src[edge_index], y = edge_num_dest[edge_index];
+    if (x > MAX_STATES || y > MAX_STATES) {
+        bp_fail("send_message: table %ux%u exceeds MAX_STATES", x, y);
+        return;
+    }
+    float J[MAX_STATES * MAX_STATES] = {0}, b[MAX_STATES] = {0}, m[MAX_STATES] = {0};
+    for (unsigned i = 0; i < x; ++i)
+        for (unsigned j = 0; j < y; ++j)
+            J[i * MAX_STATES + j] = edge_joint_probabilities[(size_t)edge_index * MAX_STATES * MAX_STATES + i * MAX_STATES + j];
+    for (unsigned j = 0; j < y; ++j)
+        b[j] = states[offset + j];
+    float *dJ = nullptr, *db = nullptr, *dm = nullptr;
+    bool ok = hipMalloc((void **)&dJ, sizeof(J)) == hipSuccess && hipMalloc((void **)&db, sizeof(b)) == hipSuccess &&
+              hipMalloc((void **)&dm, sizeof(m)) == hipSuccess &&
+              hipMemcpy(dJ, J, sizeof(J), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(db, b, sizeof(b), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_send_one<MAX_STATES>, dim3(1), dim3(64), 0, 0, dJ, db, x, y, dm);
+        ok = hipGetLastError() == hipSuccess && hipMemcpy(m, dm, sizeof(m), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    if (dJ) (void)hipFree(dJ);
+    if (db) (void)hipFree(db);
+    if (dm) (void)hipFree(dm);
+    if (!ok) {
+        bp_fail("send_message: no usable HIP device");
+        return;
+    }
+    for (unsigned i = 0; i < x; ++i)
+        edge_messages[(size_t)edge_index * MAX_STATES + i] = m[i];
+}

@@ -1,0 +1,376 @@
+/*
+ * bp_kernels.h -- gfx950 kernels of the loopy BP hot path (included by
+ * bp_device.hip only).
+ *
+ * Device layout (built by bp_upload from a Graph_t):
+ *   slot s = position in the destination-grouped edge order of the
+ *            reference (dest_nodes_to_edges_edge_list, graph.c:285-313), so
+ *            every node's in-edges are contiguous and in the order the
+ *            reference multiplies them;
+ *   in_off[V+1]      slot range of each node
+ *   slot_src[E]      source node of the slot's edge
+ *   J[E][S*S]        the edge's table, J[i][j], i = source state, j = destination state
+ *   msg[2][E][SP]    messages, double-buffered (old / new)
+ *   states[V][SP], buf[2][V][SP]  beliefs and the combine buffer
+ *                    states (.) prod(in-messages), double-buffered
+ * S = largest state count, SP = row stride; rows are zero beyond a node's
+ * state count, which is what gives mixed-state networks their defined
+ * zero-extension semantics.
+ *
+ * One loopy iteration (src/graph/graph.c:1121-1183) is ONE launch of
+ * k_loopy<ITER>: the reference's send phase is restated as a pull over each
+ * node's in-edges, which needs the source's combine buffer of this iteration
+ * -- that buffer only depends on the source's own previous state and
+ * messages, so the previous launch leaves it in buf[next].  Per wave:
+ *   1. one lane per in-slot of a chunk of nodes (<= 64 slots, <= 64 nodes):
+ *      gather buf[src], m = N(J . buf[src]), write m, |m - old| into an FP64
+ *      partial (the delta of graph.c:1300-1309);
+ *   2. the messages are staged in LDS; one lane per node multiplies them in
+ *      slot order: t = prod m (graph.c:985-1012), states = N(states * t), and
+ *      buf[next] = states (.) prod m (graph.c:851-882 of the next iteration).
+ * Every float operation is the reference's, in its order (sums from 0.0 left
+ * to right, products in dest-CSR order, NaN factors skipped, sum <= 0 -> 1,
+ * correctly rounded division); the library is built with
+ * -ffp-contract=off so no FMA changes a rounding.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bpk {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
+
+enum Mode { ITER = 0, INIT = 1, PREP = 2 };
+
+struct LoopyArgs {
+    const unsigned *in_off;      /* [V+1] */
+    const unsigned *slot_src;    /* [E] */
+    const unsigned char *slot_xy;/* [E] x | y << 4 (mixed networks only) */
+    const float *J;              /* [E][S*S] */
+    const unsigned char *nvars;  /* [V] (mixed networks only) */
+    float *states;               /* [V][SP] */
+    const float *buf_in;         /* [V][SP] gather source (ITER) */
+    float *buf_out;              /* [V][SP] */
+    const float *msg_in;         /* [E][SP] old messages (ITER) / messages (PREP) */
+    float *msg_out;              /* [E][SP] new messages (ITER, INIT) */
+    const unsigned *chunk;       /* [nchunks+1] first node of each chunk */
+    unsigned nchunks;
+    unsigned chunks_per_block;
+    double *partials;            /* [G] this launch's per-block delta partials (ITER) */
+    const unsigned *stop;        /* non-null: skip the launch once *stop != 0 */
+};
+
+template <int S> struct Stride {
+    static constexpr int value = (S <= 2) ? S : (S <= 4) ? 4 : (S <= 8) ? 8 : 10;
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Logical block index that gives each XCD a contiguous range of chunks
+ * (workgroups are dealt to the 8 XCDs round-robin). */
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G)
+{
+    if ((G & 7u) != 0u)
+        return b;
+    return (b & 7u) * (G >> 3) + (b >> 3);
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+/* m = N(J . b): m[i] = sum_j J[i][j] * b[j] for i < x, j < y (graph.c:896-909) */
+template <int S, bool MIXED>
+__device__ __forceinline__ void send(const float *__restrict__ Jrow, const float *b, unsigned x, unsigned y, float *m)
+{
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        float acc = 0.0f;
+        if (!MIXED || (unsigned)i < x) {
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+                if (!MIXED || (unsigned)j < y)
+                    acc = acc + Jrow[i * S + j] * b[j];
+            sum = sum + acc;
+        }
+        m[i] = acc;
+    }
+    if (sum <= 0.0f)
+        sum = 1.0f;
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+        m[i] = m[i] / sum;
+}
+
+/* The message of slot s into m[0..S) (zero beyond x), without side effects. */
+template <int S, bool MIXED, int MODE>
+__device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, float *m, unsigned &x)
+{
+    constexpr int SP = Stride<S>::value;
+    x = S;
+    if (MODE == PREP) {
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+            m[i] = a.msg_in[(size_t)s * SP + i];
+        return;
+    }
+    unsigned u = a.slot_src[s];
+    unsigned y = S;
+    if (MIXED) {
+        unsigned xy = a.slot_xy[s];
+        x = xy & 15u;
+        y = xy >> 4;
+    }
+    const float *src = (MODE == INIT) ? a.states : a.buf_in;
+    float b[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+        b[j] = src[(size_t)u * SP + j];
+    send<S, MIXED>(a.J + (size_t)s * S * S, b, x, y, m);
+}
+
+/* Slot phase: the message of slot s, written to msg_out (ITER, INIT) with
+ * its |new - old| added to dacc (ITER). */
+template <int S, bool MIXED, int MODE>
+__device__ __forceinline__ void slot_message(const LoopyArgs &a, unsigned s, float *m, double &dacc)
+{
+    constexpr int SP = Stride<S>::value;
+    unsigned x;
+    compute_message<S, MIXED, MODE>(a, s, m, x);
+    if (MODE == PREP)
+        return;
+    if (MODE == ITER) {
+        const float *old = a.msg_in + (size_t)s * SP;
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            if (!MIXED || (unsigned)i < x) {
+                float q = old[i] - m[i];
+                if (q != q)
+                    q = 0.0f;
+                dacc += (double)fabsf(q);
+            }
+        }
+    }
+    float *dst = a.msg_out + (size_t)s * SP;
+#pragma unroll
+    for (int i = 0; i < SP; ++i)
+        dst[i] = (i < S) ? m[i] : 0.0f;
+}
+
+/* buf[k] = buf[k] * m[k] unless m[k] is NaN (graph.c:450-459) */
+template <int S, bool MIXED>
+__device__ __forceinline__ void combine(float *buf, const float *m, unsigned n)
+{
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        if (!MIXED || (unsigned)k < n) {
+            float mk = m[k];
+            if (mk == mk)
+                buf[k] = buf[k] * mk;
+        }
+    }
+}
+
+template <int S, bool MIXED>
+__device__ __forceinline__ void normalise(float *v, unsigned n)
+{
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        if (!MIXED || (unsigned)k < n)
+            sum = sum + v[k];
+    if (sum <= 0.0f)
+        sum = 1.0f;
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        if (!MIXED || (unsigned)k < n)
+            v[k] = v[k] / sum;
+}
+
+template <int S, bool MIXED, int MODE>
+__device__ __forceinline__ void write_node(const LoopyArgs &a, unsigned v, const float *st, const float *b, unsigned n)
+{
+    constexpr int SP = Stride<S>::value;
+    if (MODE == ITER) {
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            if (!MIXED || (unsigned)k < n)
+                a.states[(size_t)v * SP + k] = st[k];
+    }
+#pragma unroll
+    for (int k = 0; k < SP; ++k)
+        a.buf_out[(size_t)v * SP + k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
+}
+
+/* A chunk of <= 64 nodes whose in-slots fit one wave. */
+template <int S, bool MIXED, int MODE>
+__device__ void chunk_narrow(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se, float *lds,
+                             unsigned lane, double &dacc)
+{
+    constexpr int SP = Stride<S>::value;
+    unsigned s = sb + lane;
+    if (s < se) {
+        float m[S];
+        slot_message<S, MIXED, MODE>(a, s, m, dacc);
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            lds[k * WAVE + lane] = m[k];
+    }
+    wave_sync();
+    unsigned v = nb + lane;
+    if (v < ne) {
+        unsigned q0 = a.in_off[v] - sb, q1 = a.in_off[v + 1] - sb;
+        unsigned n = MIXED ? a.nvars[v] : S;
+        float st[S], b[S], m[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            st[k] = a.states[(size_t)v * SP + k];
+        if (MODE == ITER) {
+            float t[S];
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+                t[k] = 1.0f;
+            for (unsigned q = q0; q < q1; ++q) {
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    m[k] = lds[k * WAVE + q];
+                combine<S, MIXED>(t, m, n);
+            }
+            if (q1 > q0) {
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    if (!MIXED || (unsigned)k < n)
+                        st[k] = st[k] * t[k];
+            }
+            normalise<S, MIXED>(st, n);
+        }
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            b[k] = st[k];
+        for (unsigned q = q0; q < q1; ++q) {
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+                m[k] = lds[k * WAVE + q];
+            combine<S, MIXED>(b, m, n);
+        }
+        write_node<S, MIXED, MODE>(a, v, st, b, n);
+    }
+    wave_sync(); /* LDS is reused by the next chunk */
+}
+
+/* One node with more than 64 in-slots: its slots are walked in blocks of 64
+ * and lane 0 multiplies in slot order.  ITER walks twice -- the product
+ * for the belief update, then the combine buffer from the updated state --
+ * recomputing the messages on the second walk instead of reading back what
+ * other lanes wrote. */
+template <int S, bool MIXED, int MODE>
+__device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *lds, unsigned lane, double &dacc,
+                          bool first, float *acc, unsigned n)
+{
+    float m[S];
+    for (unsigned base = sb; base < se; base += WAVE) {
+        unsigned s = base + lane;
+        if (s < se) {
+            if (first) {
+                slot_message<S, MIXED, MODE>(a, s, m, dacc);
+            } else {
+                unsigned x;
+                compute_message<S, MIXED, MODE>(a, s, m, x);
+            }
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+                lds[k * WAVE + lane] = m[k];
+        }
+        wave_sync();
+        if (lane == 0) {
+            unsigned cnt = (se - base < (unsigned)WAVE) ? se - base : (unsigned)WAVE;
+            for (unsigned q = 0; q < cnt; ++q) {
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    m[k] = lds[k * WAVE + q];
+                combine<S, MIXED>(acc, m, n);
+            }
+        }
+        wave_sync();
+    }
+}
+
+template <int S, bool MIXED, int MODE>
+__device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned se, float *lds, unsigned lane,
+                           double &dacc)
+{
+    constexpr int SP = Stride<S>::value;
+    unsigned n = MIXED ? a.nvars[v] : S;
+    float st[S], acc[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        st[k] = a.states[(size_t)v * SP + k];
+        acc[k] = (MODE == ITER) ? 1.0f : st[k];
+    }
+    walk_wide<S, MIXED, MODE>(a, sb, se, lds, lane, dacc, true, acc, n);
+    if (MODE == ITER) {
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            if (!MIXED || (unsigned)k < n)
+                st[k] = st[k] * acc[k];
+        normalise<S, MIXED>(st, n);
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            acc[k] = st[k];
+        walk_wide<S, MIXED, MODE>(a, sb, se, lds, lane, dacc, false, acc, n);
+    }
+    if (lane == 0)
+        write_node<S, MIXED, MODE>(a, v, st, acc, n);
+}
+
+template <int S, bool MIXED, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
+{
+    if (a.stop && *a.stop)
+        return;
+    __shared__ float lds[WAVES_PER_BLOCK][S * WAVE];
+    __shared__ double red[WAVES_PER_BLOCK];
+    unsigned wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    unsigned lb = xcd_block(blockIdx.x, gridDim.x);
+    unsigned c0 = lb * a.chunks_per_block;
+    unsigned c1 = c0 + a.chunks_per_block;
+    if (c1 > a.nchunks)
+        c1 = a.nchunks;
+    double dacc = 0.0;
+    for (unsigned c = c0 + wave; c < c1; c += WAVES_PER_BLOCK) {
+        unsigned nb = a.chunk[c], ne = a.chunk[c + 1];
+        unsigned sb = a.in_off[nb], se = a.in_off[ne];
+        if (se - sb <= (unsigned)WAVE)
+            chunk_narrow<S, MIXED, MODE>(a, nb, ne, sb, se, lds[wave], lane, dacc);
+        else
+            chunk_wide<S, MIXED, MODE>(a, nb, sb, se, lds[wave], lane, dacc);
+    }
+    if (MODE == ITER) {
+        double d = wave_sum(dacc);
+        if (lane == 0)
+            red[wave] = d;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < WAVES_PER_BLOCK; ++w)
+                t += red[w];
+            a.partials[blockIdx.x] = t;
+        }
+    }
+}
+
+} // namespace bpk

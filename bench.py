@@ -1,0 +1,165 @@
+"""Benchmark: loopy belief propagation on one MI355X.
+
+A step is one loopy BP run over the synthetic network -- init_previous_edge
+plus 20 iterations, each with its FP64 convergence delta -- on a seeded
+binary DAG of 500,000 nodes and ~1M arcs (in-degree min(Poisson(2), 6),
+CPT rows U(0,1), 2% one-hot evidence).  Inputs are resident in HBM before
+the timed region.  The metric is message-updates/s = E x iterations / time,
+E being the directed message-edge count (the reference's
+current_num_edges).
+
+Prints one JSON line (see the keys below).  The roofline object covers the
+per-iteration kernel; its duration is measured with HIP events on the
+library's stream, and its algorithmic bytes are those the iteration must
+move at minimum.  The CPU baseline times the oracle (the C restatement of
+the reference's OpenMP node loop, bit-identical to it) on this host, on the
+same network, and checks that the GPU beliefs equal it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for p in (ROOT / "belief-propagation_amd", ROOT / "oracle"):
+    sys.path.insert(0, str(p))
+
+import bp_ctypes as B  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+ITERATIONS = 20
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.txt), or None."""
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_threads() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--preset", default="c2b")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU partitioning is a design only)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    lib = B.load()
+    lib.bp_set_abort_on_error(0)
+    if not torch.cuda.is_available():
+        sys.exit("bench.py: no GPU visible")
+
+    bn = B.generate(args.preset)
+    g = B.bn_to_graph(bn)
+    G = g.contents
+    V, E = G.current_num_vertices, G.current_num_edges
+    arcs = int(np.count_nonzero(B.view_u32(G.edges_src_index, E) < B.view_u32(G.edges_dest_index, E)))
+    dev = B.Device(g, device=0)
+    info = dev.info()
+
+    def step():
+        dev.init()
+        dev.iterate(ITERATIONS)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    dev.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dev.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = E * ITERATIONS * args.steps / elapsed
+
+    # per-iteration kernel: HIP events around each launch on the library's stream
+    dev.init()
+    ms = dev.profile(ITERATIONS * 5)
+    kernel_ms = float(np.mean(ms))
+    alg = dev.algorithmic_bytes()
+    achieved = alg / (kernel_ms * 1e-3) / 1e9
+    traffic = read_traffic(f"loopy_iter_{args.preset}_traffic.txt")
+
+    # the GPU result of one full step, kept for the parity check below
+    dev.init()
+    deltas = dev.iterate(ITERATIONS, want_deltas=True)
+    dev.download(g)
+    gpu_states = B.states(g)
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        import oracle_ctypes as O
+
+        threads = cpu_threads()
+        h = B.bn_to_graph(bn)
+        t0 = time.perf_counter()
+        O.init_previous_edge(h)
+        for _ in range(ITERATIONS):
+            O.loopy_iteration(h, threads=threads)
+        t_cpu = time.perf_counter() - t0
+        parity = bool(B.states(h).tobytes() == gpu_states.tobytes())
+        cpu = {"value": E * ITERATIONS / t_cpu, "unit": "msg-updates/s", "cores": threads, "kind": "port",
+               "sample": f"the full workload once: init + {ITERATIONS} iterations of the oracle's OpenMP node loop "
+                         f"({t_cpu:.2f} s wall, {threads} threads)"}
+        lib.graph_destroy(h)
+
+    out = {
+        "metric": "edge message-updates/s at 1M edges (loopy BP, 20 iterations + FP64 delta)",
+        "value": value,
+        "unit": "msg-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{args.preset}: seeded binary DAG, init + {ITERATIONS} loopy iterations",
+                   "nodes": V, "arcs": arcs, "message_edges": E, "iterations": ITERATIONS,
+                   "state_stride": info["stride"], "delta": "fp64 per iteration"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_loopy<ITER>", "kernel_us": kernel_ms * 1e3,
+                     "algorithmic_bytes_per_launch": alg},
+        "cpu_baseline": cpu,
+        "parity_vs_cpu": parity,
+        "final_delta": float(deltas[-1]),
+    }
+    dev.free()
+    lib.graph_destroy(g)
+    lib.bp_bn_free(bn)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

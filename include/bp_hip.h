@@ -1,0 +1,119 @@
+/*
+ * bp_hip.h -- C-ABI of the MI355X (gfx950) belief-propagation engine in
+ * libbp_amd.so.  Plain C types only; every function returns 0 on success
+ * and -1 on failure with the message in bp_last_error() (it never aborts,
+ * whatever bp_set_abort_on_error() says).
+ *
+ * Two ways in:
+ *  1. The drop-in graph API of graph.h (init_previous_edge,
+ *     loopy_propagate_*, propagate_using_levels*, marginalize): each call
+ *     uploads what it reads from the Graph_t, runs on the GPU and writes its
+ *     results back, like the reference's CPU functions.
+ *  2. A device-resident handle (bp_dev) for callers that keep the graph on
+ *     the GPU across calls: upload once, iterate, download when needed.  A
+ *     handle owns one HIP stream; calls on different handles may run from
+ *     different threads.
+ *
+ * Reference interfaces replaced:
+ *   loopy_propagate_until_hip       <- loopy_propagate_until_cuda
+ *                                      (src/cuda_benchmark/belief-propagation.cu:529)
+ *   loopy_propagate_until_hip_edge  <- loopy_propagate_until_cuda_edge
+ *                                      (src/cuda_benchmark/belief-propagation.cu:664)
+ *   bp_upload / bp_download         <- the cudaMalloc + cudaMemcpy blocks of
+ *                                      belief-propagation.cu:567-600,634-636
+ *   bp_loopy_init                   <- init_previous_edge (src/graph/graph.c:725)
+ *   bp_loopy_iterate                <- loopy_propagate_one_iteration x n
+ *                                      (graph.c:1121), plus the per-iteration
+ *                                      delta of graph.c:1300-1309 in FP64
+ *   bp_loopy_until                  <- loopy_propagate_until (graph.c:1278)
+ *   bp_tree                         <- the tree driver sequence
+ *                                      (src/c_benchmark/main.c:205-215)
+ */
+#ifndef BP_HIP_H_
+#define BP_HIP_H_
+
+#include <stdint.h>
+
+#include "graph.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bp_dev bp_dev;
+
+/* Errors (shared with the drop-in API). */
+const char *bp_last_error(void);
+int bp_set_abort_on_error(int enabled); /* returns the previous setting */
+
+/* Number of HIP devices visible (0 without a GPU); never fails. */
+int bp_device_count(void);
+
+/* GPU propagation of the reference's CUDA entry point: uploads g, runs
+ * init_previous_edge and up to max_iterations loopy iterations with the
+ * stop test of graph.c:1313 evaluated on an FP64 delta after every
+ * iteration, downloads node_states and both message arrays (with the
+ * pointer swaps the CPU path would have made) and returns the reference's
+ * return value (index of the converged iteration, or max_iterations).
+ * Returns 0xffffffff on failure. */
+unsigned int loopy_propagate_until_hip(Graph_t g, float convergence, unsigned int max_iterations);
+
+/* The same through the edge-centric schedule (one pass over the edges,
+ * then one over the nodes, per iteration). */
+unsigned int loopy_propagate_until_hip_edge(Graph_t g, float convergence, unsigned int max_iterations);
+
+/* ---- device-resident handle ---------------------------------------------- */
+
+/* Uploads structure, tables, node states and both message arrays of g (its
+ * CSR views must be set up).  device < 0 selects the current device. */
+int bp_upload(Graph_t g, int device, bp_dev **out);
+void bp_free(bp_dev *d);
+
+/* Re-reads node_states and the message arrays from g (same structure). */
+int bp_upload_state(bp_dev *d, Graph_t g);
+
+/* Writes node_states, both message arrays and the message pointer swaps
+ * back into g; for tree runs, edges_messages holds the tree messages. */
+int bp_download(bp_dev *d, Graph_t g);
+
+/* Node beliefs as [V][stride] floats (stride = bp_dev_info(...)[2]). */
+int bp_download_states(bp_dev *d, float *out);
+
+/* init_previous_edge on the device. */
+int bp_loopy_init(bp_dev *d);
+
+/* n loopy iterations; deltas (nullable) receives the FP64 delta of each.
+ * The launches are replayed from a cached HIP graph. */
+int bp_loopy_iterate(bp_dev *d, unsigned int n, double *deltas);
+
+/* Like loopy_propagate_until: runs until the stop test fires or
+ * max_iterations; *result receives the reference's return value. */
+int bp_loopy_until(bp_dev *d, float convergence, unsigned int max_iterations, unsigned int *result);
+
+/* Tree BP: start, up-sweep over levels 1..L-2, down-sweep over L-1..1,
+ * marginalize.  The Graph_t given to bp_upload must have its levels set
+ * (init_levels_to_nodes). */
+int bp_tree(bp_dev *d);
+
+/* Waits for all work queued on the handle. */
+int bp_sync(bp_dev *d);
+
+/* info[0]=V, info[1]=E (message edges), info[2]=state stride,
+ * info[3]=max state count, info[4]=chunks of the node kernel,
+ * info[5]=1 if state counts are mixed, info[6]=tree sub-waves */
+int bp_dev_info(const bp_dev *d, uint64_t *info);
+
+/* Algorithmic HBM bytes of one loopy iteration of this graph (the bytes a
+ * kernel must move at minimum; see DESIGN.md of the repository):
+ *   distinct tables 4*x*y + per edge 4*(2x + y + 1) + per node (12*n + 4) */
+double bp_loopy_algorithmic_bytes(const bp_dev *d);
+
+/* Times each of n iterations with HIP events recorded on the handle's
+ * stream around its kernel launch (no graph).  ms[i] gets iteration i. */
+int bp_profile_loopy(bp_dev *d, unsigned int n, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BP_HIP_H_ */

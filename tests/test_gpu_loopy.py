@@ -1,0 +1,226 @@
+"""GPU loopy BP (libbp_amd.so) against the reference's results.
+
+The bar is bit-exactness: the kernels perform the reference's float
+operations in its order (src/graph/graph.c:426-459,851-1032), so beliefs and
+messages must equal the reference's to the last bit.  The convergence delta
+is accumulated in FP64 on the GPU; it is compared with the same sum computed
+in double by the oracle (relative 1e-12: only the order of the FP64 additions
+differs).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import bp_ctypes as B
+from conftest import (SMALL_LOOPY_CASES, bits_report, golden, graph_for_case, messages_of, same_bits,
+                      states_of)
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("case", SMALL_LOOPY_CASES)
+def test_handle_iterations_bitwise(lib, orc, case):
+    print(f"case {case}", flush=True)
+    d = golden(case)
+    g = graph_for_case(d)
+    S = d["loopy_states"].shape[-1]
+    dev = B.Device(g)
+    dev.init()
+    state_iters = list(d["loopy_state_iters"])
+    msg_iters = list(d["loopy_msg_iters"])
+    n = int(max(state_iters))
+    # the oracle, in lockstep, provides the FP64 deltas to compare with
+    h = graph_for_case(d)
+    orc.init_previous_edge(h)
+    dev.download(g)
+    assert same_bits(messages_of(g, "prev", S), d["loopy_init_prev"])
+    for it in range(1, n + 1):
+        deltas = dev.iterate(1, want_deltas=True)
+        orc.loopy_iteration(h)
+        want = orc.delta64(h)
+        assert abs(deltas[0] - want) <= 1e-12 * max(1.0, abs(want)), f"delta {it}: {deltas[0]} vs {want}"
+        # the reference's own f32 delta rounds the same sum
+        assert abs(deltas[0] - float(d["loopy_deltas"][it - 1])) <= 1e-5 * max(1.0, abs(want))
+        if it in state_iters or it in msg_iters:
+            dev.download(g)
+        if it in state_iters:
+            got = states_of(g, S)
+            ref = d["loopy_states"][state_iters.index(it)]
+            assert same_bits(got, ref), f"{case} iteration {it}: {bits_report(got, ref)}"
+        if it in msg_iters:
+            k = msg_iters.index(it)
+            assert same_bits(messages_of(g, "cur", S), d["loopy_cur"][k])
+            assert same_bits(messages_of(g, "prev", S), d["loopy_prev"][k])
+    dev.free()
+    lib.graph_destroy(g)
+    lib.graph_destroy(h)
+
+
+@pytest.mark.parametrize("case", SMALL_LOOPY_CASES)
+def test_batched_iterations_equal_single_steps(lib, case):
+    """n iterations from one cached HIP graph == the reference after n."""
+    print(f"case {case}", flush=True)
+    d = golden(case)
+    g = graph_for_case(d)
+    S = d["loopy_states"].shape[-1]
+    dev = B.Device(g)
+    dev.init()
+    n = int(d["loopy_deltas"].shape[0])
+    deltas = dev.iterate(n, want_deltas=True)
+    dev.download(g)
+    it = list(d["loopy_state_iters"])
+    assert same_bits(states_of(g, S), d["loopy_states"][it.index(n)])
+    np.testing.assert_allclose(deltas, d["loopy_deltas"].astype(np.float64), rtol=1e-5, atol=1e-7)
+    dev.free()
+    lib.graph_destroy(g)
+
+
+@pytest.mark.parametrize("case", SMALL_LOOPY_CASES)
+def test_dropin_api_bitwise(lib, case):
+    """init_previous_edge + loopy_propagate_one_iteration through graph.h."""
+    print(f"case {case}", flush=True)
+    d = golden(case)
+    g = graph_for_case(d)
+    S = d["loopy_states"].shape[-1]
+    lib.init_previous_edge(g)
+    assert same_bits(messages_of(g, "prev", S), d["loopy_init_prev"])
+    msg_iters = list(d["loopy_msg_iters"])
+    for it in range(1, max(msg_iters) + 1):
+        lib.loopy_propagate_one_iteration(g)
+        if it in msg_iters:
+            k = msg_iters.index(it)
+            assert same_bits(messages_of(g, "cur", S), d["loopy_cur"][k]), f"iteration {it}"
+            assert same_bits(messages_of(g, "prev", S), d["loopy_prev"][k]), f"iteration {it}"
+    st_iters = list(d["loopy_state_iters"])
+    if max(msg_iters) in st_iters:
+        assert same_bits(states_of(g, S), d["loopy_states"][st_iters.index(max(msg_iters))])
+    lib.graph_destroy(g)
+
+
+@pytest.mark.parametrize("case", SMALL_LOOPY_CASES)
+def test_until_bitwise(lib, case):
+    """loopy_propagate_until and loopy_propagate_until_hip: iteration count,
+    beliefs and both message arrays (with the reference's pointer swaps)."""
+    print(f"case {case}", flush=True)
+    d = golden(case)
+    S = d["until_states"].shape[-1]
+    conv, mx = float(d["until_conv"]), int(d["until_max"])
+    for fn in ("dropin", "hip"):
+        g = graph_for_case(d)
+        if fn == "dropin":
+            lib.init_previous_edge(g)
+            it = lib.loopy_propagate_until(g, conv, mx)
+        else:
+            it = lib.loopy_propagate_until_hip(g, conv, mx)
+        assert it == int(d["until_iterations"][0]), fn
+        assert same_bits(states_of(g, S), d["until_states"]), fn
+        assert same_bits(messages_of(g, "cur", S), d["until_cur"]), fn
+        assert same_bits(messages_of(g, "prev", S), d["until_prev"]), fn
+        lib.graph_destroy(g)
+
+
+def test_mixed_state_counts_match_oracle(lib, orc):
+    """Different state counts per node: zero-extension, defined by the oracle."""
+    bn = B.generate("c1", states_hi=4, seed=1002)
+    g = B.bn_to_graph(bn)
+    h = B.bn_to_graph(bn)
+    dev = B.Device(g)
+    assert dev.info()["mixed"] == 1
+    dev.init()
+    orc.init_previous_edge(h)
+    dev.iterate(15)
+    for _ in range(15):
+        orc.loopy_iteration(h)
+    dev.download(g)
+    assert same_bits(B.states(g), B.states(h))
+    assert same_bits(B.messages(g, "cur"), B.messages(h, "cur"))
+    assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
+    dev.free()
+    for x in (g, h):
+        lib.graph_destroy(x)
+    lib.bp_bn_free(bn)
+
+
+def _star(lib, n_leaves: int, states: int):
+    """A hub with n_leaves in-edges (more than one wave of slots)."""
+    import ctypes as C
+
+    rng = np.random.default_rng(n_leaves)
+    g = lib.create_graph(n_leaves + 1, 2 * n_leaves)
+    lib.graph_add_node(g, states, b"hub")
+    st = (C.c_float * 10)()
+    for i in range(n_leaves):
+        lib.graph_add_node(g, states, f"leaf{i}".encode())
+        if i % 3 == 0:
+            p = rng.random(states).astype(np.float32)
+            for k in range(states):
+                st[k] = float(p[k])
+            lib.graph_set_node_state(g, i + 1, states, st)
+    tab = (C.c_float * 100)()
+    for i in range(n_leaves):
+        J = rng.random((states, states)).astype(np.float32)
+        for a in range(states):
+            for b in range(states):
+                tab[a * 10 + b] = float(J[a, b])
+        lib.graph_add_edge(g, i + 1, 0, states, states, tab)
+        lib.graph_add_edge(g, 0, i + 1, states, states, tab)
+    lib.set_up_src_nodes_to_edges(g)
+    lib.set_up_dest_nodes_to_edges(g)
+    return g
+
+
+@pytest.mark.parametrize("n_leaves,states", [(63, 2), (64, 3), (65, 2), (200, 4), (1000, 2)])
+def test_high_in_degree_matches_oracle(lib, orc, n_leaves, states):
+    """Nodes with more in-edges than a wave has lanes (the reference caps
+    lists at MAX_DEGREE; the drop-in does not)."""
+    print(f"star {n_leaves} x {states}", flush=True)
+    g = _star(lib, n_leaves, states)
+    h = _star(lib, n_leaves, states)
+    dev = B.Device(g)
+    dev.init()
+    orc.init_previous_edge(h)
+    dev.iterate(6)
+    for _ in range(6):
+        orc.loopy_iteration(h)
+    dev.download(g)
+    assert same_bits(B.states(g), B.states(h)), bits_report(B.states(g), B.states(h))
+    assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
+    dev.free()
+    lib.graph_destroy(g)
+    lib.graph_destroy(h)
+
+
+def test_send_message_dropin(lib, orc):
+    """send_message (graph.c:426-448) on one edge, run on the GPU."""
+    import ctypes as C
+
+    d = golden("lattice16")
+    g = graph_for_case(d)
+    G = g.contents
+    E = G.current_num_edges
+    h = graph_for_case(d)
+    for e in (0, 7, E - 1):
+        src = G.edges_src_index[e]
+        lib.send_message(G.node_states, 10 * src, e, G.edges_joint_probabilities, G.edges_messages,
+                         G.edges_x_dim, G.edges_y_dim)
+        b = (C.c_float * 10)(*B.view_f32(G.node_states, G.current_num_vertices * 10)[10 * src:10 * src + 10])
+        out = (C.c_float * 10)()
+        orc.load().orc_send_message(b, G.node_num_vars[src], C.cast(h, C.c_void_p), e, out)
+        got = B.view_f32(G.edges_messages, E * 10)[10 * e:10 * e + 4]
+        assert same_bits(got, np.array(out[:4], dtype=np.float32))
+    lib.graph_destroy(g)
+    lib.graph_destroy(h)
+
+
+def test_upload_errors_are_reported(lib):
+    """Status codes, not aborts, at the C-ABI."""
+    import ctypes as C
+
+    g = lib.create_graph(3, 2)
+    lib.graph_add_node(g, 2, b"a")
+    lib.graph_add_node(g, 2, b"b")
+    h = C.c_void_p()
+    assert lib.bp_upload(g, -1, C.byref(h)) == -1  # 2 of 3 nodes added
+    assert b"nodes" in lib.bp_last_error()
+    lib.graph_destroy(g)
