@@ -146,6 +146,7 @@ _SIGNATURES = {
     "bp_download": (C.c_int, [C.c_void_p, GraphP]),
     "bp_download_states": (C.c_int, [C.c_void_p, f32p]),
     "bp_loopy_init": (C.c_int, [C.c_void_p]),
+    "bp_restore_states": (C.c_int, [C.c_void_p]),
     "bp_loopy_iterate": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_double)]),
     "bp_loopy_until": (C.c_int, [C.c_void_p, C.c_float, C.c_uint, C.POINTER(C.c_uint)]),
     "bp_tree": (C.c_int, [C.c_void_p]),
@@ -314,6 +315,9 @@ class Device:
 
     def init(self):
         check(self.lib.bp_loopy_init(self.h), "bp_loopy_init")
+
+    def restore(self):
+        check(self.lib.bp_restore_states(self.h), "bp_restore_states")
 
     def iterate(self, n: int, want_deltas: bool = False):
         if want_deltas:

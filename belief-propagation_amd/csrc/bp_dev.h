@@ -41,12 +41,13 @@ struct bp_dev {
     float *states = nullptr;          /* [V][SP] */
     float *buf[2] = {nullptr, nullptr};
     float *msg[2] = {nullptr, nullptr};
-    float *tree_msg = nullptr;        /* [E][SP] the reference's edges_messages, slot order */
-    unsigned *chunk = nullptr;
-    unsigned nchunks = 0, G = 0, cpb = 0;
+    float *states0 = nullptr;         /* [V][SP] the states given at upload, for bp_restore_states */
+    uint2 *chunk = nullptr;     /* [nchunks+1] (first node, first slot) */
+    unsigned nchunks = 0, G = 0;
+    unsigned part_cap = 1;      /* iterations whose delta partials fit the buffer */
 
-    double *partials = nullptr; /* [PART_CAP][G] */
-    double *deltas = nullptr;   /* [PART_CAP] */
+    double *partials = nullptr; /* [part_cap][G] */
+    double *deltas = nullptr;   /* [max(part_cap,16)] */
     double *last = nullptr;
     unsigned *stop = nullptr;
 
@@ -62,7 +63,11 @@ struct bp_dev {
     int bcur = 0;         /* buf[bcur] holds the combine buffer of the next iteration */
     bool buf_valid = false;
     unsigned swaps = 0;   /* iterations since upload: host pointer swaps owed */
-    bool tree_valid = false;
+    /* The host keeps two message arrays, edges_messages (A) and
+     * last_edges_messages, and swaps which one current_edge_messages points
+     * at; tree BP always works on A.  cur_is_a records whether the host's
+     * current pointer pointed at A at upload / last download. */
+    bool cur_is_a = true;
 
     std::map<unsigned long long, hipGraphExec_t> graphs;
 
@@ -84,4 +89,9 @@ template <typename T> static int dmalloc(T **p, size_t count)
 int bp_tree_plan_build(bp_dev *d, Graph_t g);
 void bp_tree_plan_free(bp_tree_plan *p);
 unsigned long long bp_tree_plan_waves(const bp_tree_plan *p);
-int bp_download_tree_msgs(bp_dev *d, Graph_t g);
+/* The device buffer mirroring the host's edges_messages array. */
+inline int a_index(const bp_dev *d)
+{
+    bool cur_points_a = d->cur_is_a ^ ((d->swaps & 1u) != 0);
+    return cur_points_a ? (d->latest ^ 1) : d->latest;
+}

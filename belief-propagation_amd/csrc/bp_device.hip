@@ -65,6 +65,13 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const double *partials, unsign
     }
 }
 
+__global__ void k_copy_floats(const float *__restrict__ src, float *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = src[i];
+}
+
 __global__ void k_until_reset(double *last, unsigned *stop)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -86,12 +93,11 @@ __global__ void k_send_one(const float *J, const float *b, unsigned x, unsigned 
 }
 
 
-static const unsigned PART_CAP = 256; /* iterations per delta batch */
 
 static void free_dev_arrays(bp_dev *d)
 {
     void *ptrs[] = {d->in_off, d->slot_src, d->slot_xy, d->J, d->nvars, d->states, d->buf[0], d->buf[1],
-                    d->msg[0], d->msg[1], d->tree_msg, d->chunk, d->partials, d->deltas, d->last, d->stop};
+                    d->msg[0], d->msg[1], d->states0, d->chunk, d->partials, d->deltas, d->last, d->stop};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -124,23 +130,24 @@ extern "C" int bp_device_count(void)
     return n;
 }
 
-/* Chunks: consecutive nodes with at most 64 in-slots and 64 nodes, or one
- * node with more in-slots. */
-static void build_chunks(const std::vector<unsigned> &in_off, unsigned V, std::vector<unsigned> &chunks)
+/* Chunks: consecutive nodes with at most cap in-slots and cap nodes (cap =
+ * 64 x slots per lane), or one node with more in-slots.  Each entry holds
+ * the chunk's first node and first slot. */
+static void build_chunks(const std::vector<unsigned> &in_off, unsigned V, unsigned cap, std::vector<uint2> &chunks)
 {
     chunks.clear();
     unsigned v = 0;
     while (v < V) {
-        chunks.push_back(v);
+        chunks.push_back(make_uint2(v, in_off[v]));
         unsigned s0 = in_off[v];
         unsigned w = v;
-        while (w < V && w - v < (unsigned)WAVE && in_off[w + 1] - s0 <= (unsigned)WAVE)
+        while (w < V && w - v < cap && in_off[w + 1] - s0 <= cap)
             ++w;
         if (w == v)
-            w = v + 1; /* a node with more than 64 in-slots gets a chunk of its own */
+            w = v + 1; /* a node with more in-slots than a chunk holds gets one of its own */
         v = w;
     }
-    chunks.push_back(V);
+    chunks.push_back(make_uint2(V, in_off[V]));
 }
 
 static int check_graph(Graph_t g)
@@ -207,6 +214,7 @@ static int upload_state(bp_dev *d, Graph_t g, bool tables)
     /* node states */
     pack_rows(g->node_states, d->node_n.data(), d->V, d->SP, tmp);
     HIP_TRY(hipMemcpyAsync(d->states, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemcpyAsync(d->states0, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
     /* messages in slot order: msg[0] = previous, msg[1] = current */
     std::vector<unsigned> slot_x(d->E);
@@ -218,9 +226,7 @@ static int upload_state(bp_dev *d, Graph_t g, bool tables)
     pack_rows(*g->current_edge_messages, slot_x.data(), d->E, d->SP, tmp, d->slot_edge.data());
     HIP_TRY(hipMemcpyAsync(d->msg[1], tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
-    pack_rows(g->edges_messages, slot_x.data(), d->E, d->SP, tmp, d->slot_edge.data());
-    HIP_TRY(hipMemcpyAsync(d->tree_msg, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
-    HIP_TRY(hipStreamSynchronize(d->stream));
+    d->cur_is_a = (g->current_edge_messages == &g->edges_messages);
     if (tables) {
         unsigned S = d->S;
         tmp.assign((size_t)d->E * S * S, 0.0f);
@@ -239,7 +245,6 @@ static int upload_state(bp_dev *d, Graph_t g, bool tables)
     d->bcur = 0;
     d->buf_valid = false;
     d->swaps = 0;
-    d->tree_valid = false;
     return 0;
 }
 
@@ -339,25 +344,24 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
         slot_src[s] = g->edges_src_index[e];
         slot_xy[s] = (unsigned char)(g->edges_x_dim[e] | (g->edges_y_dim[e] << 4));
     }
-    std::vector<unsigned> chunks;
-    build_chunks(in_off, V, chunks);
+    std::vector<uint2> chunks;
+    build_chunks(in_off, V, WAVE * (S <= 4 ? 2u : 1u), chunks);
     d->nchunks = (unsigned)chunks.size() - 1;
-    /* grid: a multiple of 8 blocks (one range per XCD), at most 2048 */
+    /* one chunk per wave; a multiple of 8 blocks so each XCD gets one range */
     unsigned want = (d->nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-    unsigned G = std::min(2048u, std::max(8u, (want + 7u) & ~7u));
+    unsigned G = std::max(8u, (want + 7u) & ~7u);
     d->G = G;
-    d->cpb = (d->nchunks + G - 1) / G;
-    if (d->cpb == 0)
-        d->cpb = 1;
+    /* delta partials of up to part_cap iterations are reduced per batch */
+    d->part_cap = (unsigned)std::min<size_t>(256, std::max<size_t>(1, (64u << 20) / (8ull * G)));
     d->alg_bytes = algorithmic_bytes(g);
 
     size_t SP = d->SP;
     if (dmalloc(&d->in_off, V + 1) || dmalloc(&d->slot_src, E) || dmalloc(&d->slot_xy, E) ||
         dmalloc(&d->J, (size_t)E * S * S) || dmalloc(&d->nvars, V) || dmalloc(&d->states, V * SP) ||
         dmalloc(&d->buf[0], V * SP) || dmalloc(&d->buf[1], V * SP) || dmalloc(&d->msg[0], (size_t)E * SP) ||
-        dmalloc(&d->msg[1], (size_t)E * SP) || dmalloc(&d->tree_msg, (size_t)E * SP) ||
-        dmalloc(&d->chunk, chunks.size()) || dmalloc(&d->partials, (size_t)PART_CAP * G) ||
-        dmalloc(&d->deltas, PART_CAP) || dmalloc(&d->last, 1) || dmalloc(&d->stop, 1)) {
+        dmalloc(&d->msg[1], (size_t)E * SP) || dmalloc(&d->states0, V * SP) ||
+        dmalloc(&d->chunk, chunks.size()) || dmalloc(&d->partials, (size_t)d->part_cap * G) ||
+        dmalloc(&d->deltas, std::max(d->part_cap, 16u)) || dmalloc(&d->last, 1) || dmalloc(&d->stop, 1)) {
         bp_free(d);
         return -1;
     }
@@ -368,7 +372,7 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
               hipMemcpy(d->slot_src, slot_src.data(), (size_t)E * 4, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(d->slot_xy, slot_xy.data(), E, hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(d->nvars, nv.data(), V, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(d->chunk, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d->chunk, chunks.data(), chunks.size() * sizeof(uint2), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemset(d->buf[0], 0, V * SP * 4) == hipSuccess && hipMemset(d->buf[1], 0, V * SP * 4) == hipSuccess &&
               hipMemset(d->stop, 0, 4) == hipSuccess;
     if (!ok) {
@@ -418,7 +422,6 @@ static LoopyArgs make_args(bp_dev *d)
     a.msg_out = d->msg[d->latest ^ 1];
     a.chunk = d->chunk;
     a.nchunks = d->nchunks;
-    a.chunks_per_block = d->cpb;
     a.partials = d->partials;
     a.stop = nullptr;
     return a;
@@ -521,7 +524,7 @@ extern "C" int bp_loopy_iterate(bp_dev *d, unsigned int n, double *deltas)
         return -1;
     unsigned done = 0;
     while (done < n) {
-        unsigned batch = std::min(n - done, PART_CAP);
+        unsigned batch = std::min(n - done, d->part_cap);
         unsigned long long key = ((unsigned long long)batch << 2) | ((unsigned)d->latest << 1) | (unsigned)d->bcur;
         auto it = d->graphs.find(key);
         if (it == d->graphs.end()) {
@@ -609,6 +612,24 @@ extern "C" int bp_loopy_until(bp_dev *d, float convergence, unsigned int max_ite
     return 0;
 }
 
+extern "C" int bp_restore_states(bp_dev *d)
+{
+    bp_clear_error();
+    if (!d) {
+        bp_set_error("bp_restore_states: NULL handle");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    size_t n = (size_t)d->V * d->SP;
+    if (n) {
+        hipLaunchKernelGGL(k_copy_floats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d->stream, d->states0,
+                           d->states, n);
+        HIP_TRY(hipGetLastError());
+    }
+    d->buf_valid = false;
+    return 0;
+}
+
 extern "C" int bp_sync(bp_dev *d)
 {
     bp_clear_error();
@@ -636,7 +657,7 @@ extern "C" int bp_profile_loopy(bp_dev *d, unsigned int n, float *ms)
         HIP_TRY(hipEventCreate(&e));
     unsigned done = 0;
     while (done < n) {
-        unsigned batch = std::min(n - done, PART_CAP);
+        unsigned batch = std::min(n - done, d->part_cap);
         for (unsigned i = 0; i < batch; ++i) {
             LoopyArgs a = make_args(d);
             a.partials = d->partials + (size_t)i * d->G;
@@ -706,12 +727,11 @@ extern "C" int bp_download(bp_dev *d, Graph_t g)
         g->current_edge_messages = t;
     }
     d->swaps = 0;
+    d->cur_is_a = (g->current_edge_messages == &g->edges_messages);
     /* the device buffers keep their roles (cached HIP graphs hold their
      * addresses); msg[latest] is the newest, i.e. the host's previous array */
     if (download_msgs(d, d->msg[d->latest], *g->previous_edge_messages) ||
         download_msgs(d, d->msg[d->latest ^ 1], *g->current_edge_messages))
-        return -1;
-    if (d->tree_valid && download_msgs(d, d->tree_msg, g->edges_messages))
         return -1;
     return 0;
 }

@@ -22,7 +22,8 @@
  * node's in-edges, which needs the source's combine buffer of this iteration
  * -- that buffer only depends on the source's own previous state and
  * messages, so the previous launch leaves it in buf[next].  Per wave:
- *   1. one lane per in-slot of a chunk of nodes (<= 64 slots, <= 64 nodes):
+ *   1. one lane per in-slot of a chunk of nodes (<= 64 R slots and nodes, R
+ *      slots per lane):
  *      gather buf[src], m = N(J . buf[src]), write m, |m - old| into an FP64
  *      partial (the delta of graph.c:1300-1309);
  *   2. the messages are staged in LDS; one lane per node multiplies them in
@@ -57,9 +58,8 @@ struct LoopyArgs {
     float *buf_out;              /* [V][SP] */
     const float *msg_in;         /* [E][SP] old messages (ITER) / messages (PREP) */
     float *msg_out;              /* [E][SP] new messages (ITER, INIT) */
-    const unsigned *chunk;       /* [nchunks+1] first node of each chunk */
+    const uint2 *chunk;          /* [nchunks+1] (first node, first slot) of each chunk */
     unsigned nchunks;
-    unsigned chunks_per_block;
     double *partials;            /* [G] this launch's per-block delta partials (ITER) */
     const unsigned *stop;        /* non-null: skip the launch once *stop != 0 */
 };
@@ -216,60 +216,141 @@ __device__ __forceinline__ void write_node(const LoopyArgs &a, unsigned v, const
         a.buf_out[(size_t)v * SP + k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
 }
 
-/* A chunk of <= 64 nodes whose in-slots fit one wave. */
-template <int S, bool MIXED, int MODE>
-__device__ void chunk_narrow(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se, float *lds,
-                             unsigned lane, double &dacc)
+/* Row loads: 8- and 16-byte vector loads where the row size allows. */
+template <int N>
+__device__ __forceinline__ void load_row(const float *__restrict__ p, float *out)
+{
+    if constexpr (N % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i) {
+            float4 v = reinterpret_cast<const float4 *>(p)[i];
+            out[4 * i] = v.x;
+            out[4 * i + 1] = v.y;
+            out[4 * i + 2] = v.z;
+            out[4 * i + 3] = v.w;
+        }
+    } else if constexpr (N % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) {
+            float2 v = reinterpret_cast<const float2 *>(p)[i];
+            out[2 * i] = v.x;
+            out[2 * i + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            out[i] = p[i];
+    }
+}
+
+/* A chunk of at most 64*R nodes with at most 64*R in-slots: lane l takes
+ * slots sb+l+64r and nodes nb+l+64r (r < R).  Every load that does not
+ * depend on another one is issued before any is consumed, so a wave keeps
+ * R slots' and R nodes' worth of reads in flight across one round trip. */
+template <int S, bool MIXED, int MODE, int R>
+__device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se,
+                                             float *lds, unsigned lane, double &dacc)
 {
     constexpr int SP = Stride<S>::value;
-    unsigned s = sb + lane;
-    if (s < se) {
-        float m[S];
-        slot_message<S, MIXED, MODE>(a, s, m, dacc);
+    constexpr int W = WAVE * R;
+    unsigned u[R], xy[R], q0[R], q1[R], nn[R];
+    float J[R][S * S], old[R][S], st[R][S];
 #pragma unroll
-        for (int k = 0; k < S; ++k)
-            lds[k * WAVE + lane] = m[k];
+    for (int r = 0; r < R; ++r) {
+        unsigned s = sb + r * WAVE + lane;
+        if (s < se) {
+            if (MODE != PREP) {
+                u[r] = a.slot_src[s];
+                load_row<S * S>(a.J + (size_t)s * S * S, J[r]);
+                xy[r] = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
+            }
+            if (MODE != INIT)
+                load_row<S>(a.msg_in + (size_t)s * SP, old[r]);
+        }
+        unsigned v = nb + r * WAVE + lane;
+        if (v < ne) {
+            q0[r] = a.in_off[v] - sb;
+            q1[r] = a.in_off[v + 1] - sb;
+            load_row<S>(a.states + (size_t)v * SP, st[r]);
+            nn[r] = MIXED ? a.nvars[v] : S;
+        }
+    }
+    /* slot phase: gather, send, write, delta, stage in LDS */
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        unsigned s = sb + r * WAVE + lane;
+        if (s < se) {
+            float m[S];
+            if (MODE == PREP) {
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    m[k] = old[r][k];
+            } else {
+                const float *src = (MODE == INIT) ? a.states : a.buf_in;
+                float b[S];
+                load_row<S>(src + (size_t)u[r] * SP, b);
+                unsigned x = xy[r] & 15u, y = xy[r] >> 4;
+                send<S, MIXED>(J[r], b, x, y, m);
+                if (MODE == ITER) {
+#pragma unroll
+                    for (int i = 0; i < S; ++i) {
+                        if (!MIXED || (unsigned)i < x) {
+                            float q = old[r][i] - m[i];
+                            if (q != q)
+                                q = 0.0f;
+                            dacc += (double)fabsf(q);
+                        }
+                    }
+                }
+                float *dst = a.msg_out + (size_t)s * SP;
+#pragma unroll
+                for (int i = 0; i < SP; ++i)
+                    dst[i] = (i < S) ? m[i] : 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+                lds[k * W + r * WAVE + lane] = m[k];
+        }
     }
     wave_sync();
-    unsigned v = nb + lane;
-    if (v < ne) {
-        unsigned q0 = a.in_off[v] - sb, q1 = a.in_off[v + 1] - sb;
-        unsigned n = MIXED ? a.nvars[v] : S;
-        float st[S], b[S], m[S];
+    /* node phase: one lane per node multiplies its in-messages in slot order */
 #pragma unroll
-        for (int k = 0; k < S; ++k)
-            st[k] = a.states[(size_t)v * SP + k];
-        if (MODE == ITER) {
-            float t[S];
-#pragma unroll
-            for (int k = 0; k < S; ++k)
-                t[k] = 1.0f;
-            for (unsigned q = q0; q < q1; ++q) {
+    for (int r = 0; r < R; ++r) {
+        unsigned v = nb + r * WAVE + lane;
+        if (v < ne) {
+            unsigned n = nn[r];
+            float m[S], b[S];
+            if (MODE == ITER) {
+                float t[S];
 #pragma unroll
                 for (int k = 0; k < S; ++k)
-                    m[k] = lds[k * WAVE + q];
-                combine<S, MIXED>(t, m, n);
-            }
-            if (q1 > q0) {
+                    t[k] = 1.0f;
+                for (unsigned q = q0[r]; q < q1[r]; ++q) {
 #pragma unroll
-                for (int k = 0; k < S; ++k)
-                    if (!MIXED || (unsigned)k < n)
-                        st[k] = st[k] * t[k];
-            }
-            normalise<S, MIXED>(st, n);
-        }
+                    for (int k = 0; k < S; ++k)
+                        m[k] = lds[k * W + q];
+                    combine<S, MIXED>(t, m, n);
+                }
+                if (q1[r] > q0[r]) {
 #pragma unroll
-        for (int k = 0; k < S; ++k)
-            b[k] = st[k];
-        for (unsigned q = q0; q < q1; ++q) {
+                    for (int k = 0; k < S; ++k)
+                        if (!MIXED || (unsigned)k < n)
+                            st[r][k] = st[r][k] * t[k];
+                }
+                normalise<S, MIXED>(st[r], n);
+            }
 #pragma unroll
             for (int k = 0; k < S; ++k)
-                m[k] = lds[k * WAVE + q];
-            combine<S, MIXED>(b, m, n);
+                b[k] = st[r][k];
+            for (unsigned q = q0[r]; q < q1[r]; ++q) {
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    m[k] = lds[k * W + q];
+                combine<S, MIXED>(b, m, n);
+            }
+            write_node<S, MIXED, MODE>(a, v, st[r], b, n);
         }
-        write_node<S, MIXED, MODE>(a, v, st, b, n);
     }
-    wave_sync(); /* LDS is reused by the next chunk */
 }
 
 /* One node with more than 64 in-slots: its slots are walked in blocks of 64
@@ -337,25 +418,27 @@ __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned
         write_node<S, MIXED, MODE>(a, v, st, acc, n);
 }
 
+/* Slots per lane of the narrow path: two for small state counts. */
+template <int S> struct SlotsPerLane {
+    static constexpr int value = (S <= 4) ? 2 : 1;
+};
+
 template <int S, bool MIXED, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
 {
+    constexpr int R = SlotsPerLane<S>::value;
     if (a.stop && *a.stop)
         return;
-    __shared__ float lds[WAVES_PER_BLOCK][S * WAVE];
+    __shared__ float lds[WAVES_PER_BLOCK][S * WAVE * R];
     __shared__ double red[WAVES_PER_BLOCK];
-    unsigned wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    unsigned lb = xcd_block(blockIdx.x, gridDim.x);
-    unsigned c0 = lb * a.chunks_per_block;
-    unsigned c1 = c0 + a.chunks_per_block;
-    if (c1 > a.nchunks)
-        c1 = a.nchunks;
+    unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
+    unsigned c = xcd_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wave;
     double dacc = 0.0;
-    for (unsigned c = c0 + wave; c < c1; c += WAVES_PER_BLOCK) {
-        unsigned nb = a.chunk[c], ne = a.chunk[c + 1];
-        unsigned sb = a.in_off[nb], se = a.in_off[ne];
-        if (se - sb <= (unsigned)WAVE)
-            chunk_narrow<S, MIXED, MODE>(a, nb, ne, sb, se, lds[wave], lane, dacc);
+    if (c < a.nchunks) {
+        uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
+        unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
+        if (se - sb <= (unsigned)(WAVE * R))
+            chunk_narrow<S, MIXED, MODE, R>(a, nb, ne, sb, se, lds[wave], lane, dacc);
         else
             chunk_wide<S, MIXED, MODE>(a, nb, sb, se, lds[wave], lane, dacc);
     }

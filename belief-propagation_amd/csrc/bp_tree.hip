@@ -34,8 +34,7 @@ struct bp_tree_plan {
     unsigned *act_slot = nullptr; /* active out-slots */
     std::vector<unsigned> wave_begin; /* [waves+1] entry ranges */
     std::vector<unsigned char> wave_start;
-    std::vector<unsigned> level_nodes_marked; /* nodes whose visited flag the run sets */
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr}; /* per message buffer it runs on */
 };
 
 struct TreeArgs {
@@ -276,8 +275,9 @@ void bp_tree_plan_free(bp_tree_plan *p)
 {
     if (!p)
         return;
-    if (p->graph)
-        (void)hipGraphExecDestroy(p->graph);
+    for (hipGraphExec_t &x : p->graph)
+        if (x)
+            (void)hipGraphExecDestroy(x);
     if (p->entries)
         (void)hipFree(p->entries);
     if (p->act_off)
@@ -320,7 +320,7 @@ static TreeArgs tree_args(bp_dev *d, const bp_tree_plan *p)
     a.J = d->J;
     a.nvars = d->nvars;
     a.states = d->states;
-    a.msg = d->tree_msg;
+    a.msg = d->msg[a_index(d)];
     a.entries = p->entries;
     a.act_off = p->act_off;
     a.act_slot = p->act_slot;
@@ -342,7 +342,7 @@ template <int S, bool MIXED> static void marg_s(bp_dev *d)
     if (d->V == 0)
         return;
     hipLaunchKernelGGL((k_tree_marginalize<S, MIXED>), dim3((d->V + 255) / 256), dim3(256), 0, d->stream, d->in_off,
-                       d->nvars, d->states, d->tree_msg, d->V);
+                       d->nvars, d->states, d->msg[a_index(d)], d->V);
 }
 
 template <bool MIXED> static void dispatch(bp_dev *d, const bp_tree_plan *p, bool marginal)
@@ -394,21 +394,23 @@ extern "C" int bp_tree(bp_dev *d)
         return -1;
     }
     HIP_TRY(hipSetDevice(d->device));
-    if (!d->tree->graph) {
+    /* tree BP works on the buffer mirroring the host's edges_messages */
+    hipGraphExec_t &exec = d->tree->graph[a_index(d)];
+    if (!exec) {
         hipGraph_t graph;
         HIP_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
         enqueue(d, d->tree, true);
         HIP_TRY(hipStreamEndCapture(d->stream, &graph));
-        hipError_t e = hipGraphInstantiate(&d->tree->graph, graph, nullptr, nullptr, 0);
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) {
-            d->tree->graph = nullptr;
+            exec = nullptr;
             bp_set_error("hipGraphInstantiate failed: %s", hipGetErrorString(e));
             return -1;
         }
     }
-    HIP_TRY(hipGraphLaunch(d->tree->graph, d->stream));
-    d->tree_valid = true;
+    HIP_TRY(hipGraphLaunch(exec, d->stream));
+    d->buf_valid = false; /* messages and states changed under the loopy combine buffer */
     return 0;
 }
 
@@ -459,7 +461,7 @@ static void run_partial(Graph_t g, const char *who, int mode, unsigned level)
         bp_fail("%s: %s", who, hipGetErrorString(e));
         return;
     }
-    d->tree_valid = true;
+    d->buf_valid = false;
     if (bp_download(d, g)) {
         bp_fail("%s: %s", who, bp_last_error());
         return;

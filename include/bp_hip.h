@@ -79,6 +79,10 @@ int bp_download(bp_dev *d, Graph_t g);
 /* Node beliefs as [V][stride] floats (stride = bp_dev_info(...)[2]). */
 int bp_download_states(bp_dev *d, float *out);
 
+/* Restores the node states given to bp_upload / bp_upload_state (device to
+ * device), so a run can start over without a host transfer. */
+int bp_restore_states(bp_dev *d);
+
 /* init_previous_edge on the device. */
 int bp_loopy_init(bp_dev *d);
 
@@ -103,9 +107,12 @@ int bp_sync(bp_dev *d);
  * info[5]=1 if state counts are mixed, info[6]=tree sub-waves */
 int bp_dev_info(const bp_dev *d, uint64_t *info);
 
-/* Algorithmic HBM bytes of one loopy iteration of this graph (the bytes a
- * kernel must move at minimum; see DESIGN.md of the repository):
- *   distinct tables 4*x*y + per edge 4*(2x + y + 1) + per node (12*n + 4) */
+/* Algorithmic HBM bytes of one loopy iteration of this graph, the bytes an
+ * iteration must move at minimum:
+ *   distinct tables 4*x*y (a transposed reverse pair counts once)
+ *   + per edge 4*(2x + y + 1): message write, previous message read (the
+ *     delta), source buffer gather, source index
+ *   + per node 12*n + 4: belief read and write, combine-buffer write, offset */
 double bp_loopy_algorithmic_bytes(const bp_dev *d);
 
 /* Times each of n iterations with HIP events recorded on the handle's
