@@ -134,6 +134,12 @@ _SIGNATURES = {
     # xml-expression.h / expression.h
     "parse_xml_file": (GraphP, [C.c_char_p]),
     "parse_bif_file": (GraphP, [C.c_char_p]),
+    "parse_bif_expression": (C.c_void_p, [C.c_char_p]),
+    "parse_bif_string": (C.c_void_p, [C.c_char_p]),
+    "build_graph": (GraphP, [C.c_void_p]),
+    "create_expression": (C.c_void_p, [C.c_int, C.c_void_p, C.c_void_p]),
+    "delete_expression": (None, [C.c_void_p]),
+    "print_expression": (None, [C.c_void_p]),
     # bp_hip.h
     "bp_last_error": (C.c_char_p, []),
     "bp_set_abort_on_error": (C.c_int, [C.c_int]),
@@ -149,6 +155,8 @@ _SIGNATURES = {
     "bp_restore_states": (C.c_int, [C.c_void_p]),
     "bp_loopy_iterate": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_double)]),
     "bp_loopy_until": (C.c_int, [C.c_void_p, C.c_float, C.c_uint, C.POINTER(C.c_uint)]),
+    "bp_edge_iterate": (C.c_int, [C.c_void_p, C.c_uint]),
+    "bp_edge_until": (C.c_int, [C.c_void_p, C.c_float, C.c_uint, C.POINTER(C.c_uint)]),
     "bp_tree": (C.c_int, [C.c_void_p]),
     "bp_sync": (C.c_int, [C.c_void_p]),
     "bp_dev_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),

@@ -416,6 +416,7 @@ static LoopyArgs make_args(bp_dev *d)
     a.J = d->J;
     a.nvars = d->nvars;
     a.states = d->states;
+    a.states_out = d->states;
     a.buf_in = d->buf[d->bcur];
     a.buf_out = d->buf[d->bcur ^ 1];
     a.msg_in = d->msg[d->latest];
@@ -434,6 +435,8 @@ template <int S, bool MIXED> static void launch_s(int mode, const LoopyArgs &a, 
         hipLaunchKernelGGL((k_loopy<S, MIXED, ITER>), grid, block, 0, st, a);
     else if (mode == INIT)
         hipLaunchKernelGGL((k_loopy<S, MIXED, INIT>), grid, block, 0, st, a);
+    else if (mode == EDGE)
+        hipLaunchKernelGGL((k_loopy<S, MIXED, EDGE>), grid, block, 0, st, a);
     else
         hipLaunchKernelGGL((k_loopy<S, MIXED, PREP>), grid, block, 0, st, a);
 }
@@ -556,6 +559,166 @@ extern "C" int bp_loopy_iterate(bp_dev *d, unsigned int n, double *deltas)
     return 0;
 }
 
+/* Buffer roles while a run is being enqueued.  ITER reads msg[latest] and
+ * writes msg[latest^1], then the host's pointers swap (graph.c:1179-1182):
+ * latest flips and a swap is owed.  EDGE copies current into previous and
+ * writes current without a swap (graph.c:1186-1227): it reads msg[latest^1],
+ * writes msg[latest] and relabels; its beliefs ping-pong between states and
+ * buf[0] because every node reads its sources' beliefs of the same step. */
+struct Roles {
+    int latest, bcur;
+    bool scratch; /* EDGE: the current beliefs are in buf[0] */
+};
+
+static LoopyArgs step_args(bp_dev *d, int mode, Roles &r)
+{
+    LoopyArgs a = make_args(d);
+    if (mode == ITER) {
+        a.msg_in = d->msg[r.latest];
+        a.msg_out = d->msg[r.latest ^ 1];
+        a.buf_in = d->buf[r.bcur];
+        a.buf_out = d->buf[r.bcur ^ 1];
+        r.bcur ^= 1;
+    } else {
+        a.msg_in = d->msg[r.latest ^ 1];
+        a.msg_out = d->msg[r.latest];
+        a.states = r.scratch ? d->buf[0] : d->states;
+        a.states_out = r.scratch ? d->states : d->buf[0];
+        r.scratch = !r.scratch;
+    }
+    r.latest ^= 1;
+    return a;
+}
+
+static int commit_roles(bp_dev *d, int mode, const Roles &r, unsigned ran)
+{
+    d->latest = r.latest;
+    d->bcur = r.bcur;
+    if (mode == ITER) {
+        d->swaps += ran;
+        return 0;
+    }
+    d->buf_valid = false;
+    if (r.scratch) {
+        size_t n = (size_t)d->V * d->SP;
+        if (n)
+            hipLaunchKernelGGL(k_copy_floats, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, d->stream, d->buf[0],
+                               d->states, n);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+/* Runs until the stop test of graph.c:1313 fires on an FP64 delta after an
+ * iteration, or max_iterations; the test runs on the device, later
+ * launches of the batch return at once.  *result: the index of the
+ * converged iteration, or max_iterations (graph.c:1278-1324). */
+static int run_until(bp_dev *d, int mode, float convergence, unsigned max_iterations, unsigned *result,
+                     double *tail /* [2]: the last two deltas, nullable */)
+{
+    if (mode == ITER && ensure_buffer(d))
+        return -1;
+    hipLaunchKernelGGL(k_until_reset, dim3(1), dim3(64), 0, d->stream, d->last, d->stop);
+    HIP_TRY(hipGetLastError());
+    const unsigned batch = 16;
+    unsigned done = 0, stop = 0;
+    double hist[2] = {-1.0, 0.0};
+    std::vector<double> ds(batch);
+    Roles r = {d->latest, d->bcur, false};
+    while (done < max_iterations) {
+        unsigned n = std::min(batch, max_iterations - done);
+        Roles before = r;
+        for (unsigned i = 0; i < n; ++i) {
+            LoopyArgs a = step_args(d, mode, r);
+            a.partials = d->partials;
+            a.stop = d->stop;
+            launch(d, mode, a);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, d->stream, d->partials, d->G, 1u, d->deltas,
+                               d->last, d->stop, convergence, 1, i);
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&stop, d->stop, sizeof(unsigned), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipMemcpyAsync(ds.data(), d->deltas, n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        unsigned ran = stop ? stop : n; /* launches after the stop returned at once */
+        r = before;
+        for (unsigned i = 0; i < ran; ++i)
+            (void)step_args(d, mode, r);
+        for (unsigned i = 0; i < ran; ++i) {
+            hist[0] = hist[1];
+            hist[1] = ds[i];
+        }
+        if (commit_roles(d, mode, r, ran))
+            return -1;
+        done += ran;
+        if (stop)
+            break;
+    }
+    hipLaunchKernelGGL(k_until_reset, dim3(1), dim3(64), 0, d->stream, d->last, d->stop);
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    *result = stop ? done - 1 : max_iterations;
+    if (tail) {
+        tail[0] = hist[0];
+        tail[1] = hist[1];
+    }
+    return 0;
+}
+
+/* The OpenACC drivers' schedule (graph.c:1326-1420,1438-1520): the stop test
+ * after every BATCH_SIZE iterations on the last delta; *result counts the
+ * iterations of the batches before the converged one. */
+static int run_batched(bp_dev *d, int mode, float convergence, unsigned max_iterations, unsigned *result)
+{
+    if (mode == ITER && ensure_buffer(d))
+        return -1;
+    double previous = -1.0, delta = 0.0;
+    unsigned num_iter = 0, i;
+    Roles r = {d->latest, d->bcur, false};
+    unsigned ran = 0;
+    for (i = 0; i < max_iterations; i += BATCH_SIZE) {
+        for (unsigned j = 0; j < BATCH_SIZE; ++j) {
+            LoopyArgs a = step_args(d, mode, r);
+            a.partials = d->partials;
+            launch(d, mode, a);
+            hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, d->stream, d->partials, d->G, 1u, d->deltas + j,
+                               d->last, d->stop, 0.0f, 0, 0u);
+        }
+        ran += BATCH_SIZE;
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(&delta, d->deltas + BATCH_SIZE - 1, sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        double change = delta - previous;
+        if (change < 0.0)
+            change = -change;
+        if (delta < (double)convergence || change < (double)convergence)
+            break;
+        previous = delta;
+        num_iter += BATCH_SIZE;
+    }
+    if (commit_roles(d, mode, r, ran))
+        return -1;
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    if (i >= max_iterations)
+        printf("No Convergence: previous: %f vs current: %f\n", previous, delta);
+    *result = num_iter;
+    return 0;
+}
+
+/* n iterations of `mode` with no stop test (deltas still reduced). */
+static int run_fixed(bp_dev *d, int mode, unsigned n)
+{
+    if (mode == ITER && ensure_buffer(d))
+        return -1;
+    Roles r = {d->latest, d->bcur, false};
+    for (unsigned i = 0; i < n; ++i) {
+        LoopyArgs a = step_args(d, mode, r);
+        a.partials = d->partials;
+        launch(d, mode, a);
+    }
+    HIP_TRY(hipGetLastError());
+    return commit_roles(d, mode, r, n);
+}
+
 extern "C" int bp_loopy_until(bp_dev *d, float convergence, unsigned int max_iterations, unsigned int *result)
 {
     bp_clear_error();
@@ -564,52 +727,29 @@ extern "C" int bp_loopy_until(bp_dev *d, float convergence, unsigned int max_ite
         return -1;
     }
     HIP_TRY(hipSetDevice(d->device));
-    if (ensure_buffer(d))
+    return run_until(d, ITER, convergence, max_iterations, result, nullptr);
+}
+
+extern "C" int bp_edge_iterate(bp_dev *d, unsigned int n)
+{
+    bp_clear_error();
+    if (!d) {
+        bp_set_error("bp_edge_iterate: NULL handle");
         return -1;
-    hipLaunchKernelGGL(k_until_reset, dim3(1), dim3(64), 0, d->stream, d->last, d->stop);
-    HIP_TRY(hipGetLastError());
-    const unsigned batch = 16;
-    unsigned done = 0;
-    unsigned stop = 0;
-    while (done < max_iterations) {
-        unsigned n = std::min(batch, max_iterations - done);
-        int latest = d->latest, bcur = d->bcur;
-        for (unsigned i = 0; i < n; ++i) {
-            LoopyArgs a = make_args(d);
-            a.msg_in = d->msg[latest];
-            a.msg_out = d->msg[latest ^ 1];
-            a.buf_in = d->buf[bcur];
-            a.buf_out = d->buf[bcur ^ 1];
-            a.partials = d->partials;
-            a.stop = d->stop;
-            launch(d, ITER, a);
-            hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, d->stream, d->partials, d->G, 1u, d->deltas,
-                               d->last, d->stop, convergence, 1, i);
-            latest ^= 1;
-            bcur ^= 1;
-        }
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(&stop, d->stop, sizeof(unsigned), hipMemcpyDeviceToHost, d->stream));
-        HIP_TRY(hipStreamSynchronize(d->stream));
-        unsigned ran = stop ? stop : n; /* launches after the stop returned at once */
-        if (ran & 1u) {
-            d->latest ^= 1;
-            d->bcur ^= 1;
-        }
-        d->swaps += ran;
-        done += ran;
-        if (stop) {
-            hipLaunchKernelGGL(k_until_reset, dim3(1), dim3(64), 0, d->stream, d->last, d->stop);
-            HIP_TRY(hipStreamSynchronize(d->stream));
-            *result = done - 1; /* the reference returns the index of the converged iteration */
-            return 0;
-        }
-        /* the next batch continues the same run: keep *last, clear nothing */
     }
-    hipLaunchKernelGGL(k_until_reset, dim3(1), dim3(64), 0, d->stream, d->last, d->stop);
-    HIP_TRY(hipStreamSynchronize(d->stream));
-    *result = max_iterations;
-    return 0;
+    HIP_TRY(hipSetDevice(d->device));
+    return run_fixed(d, EDGE, n);
+}
+
+extern "C" int bp_edge_until(bp_dev *d, float convergence, unsigned int max_iterations, unsigned int *result)
+{
+    bp_clear_error();
+    if (!d || !result) {
+        bp_set_error("bp_edge_until: NULL argument");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    return run_until(d, EDGE, convergence, max_iterations, result, nullptr);
 }
 
 extern "C" int bp_restore_states(bp_dev *d)
@@ -804,35 +944,82 @@ extern "C" void loopy_propagate_one_iteration(Graph_t g)
         bp_fail("loopy_propagate_one_iteration: %s", bp_last_error());
 }
 
-extern "C" unsigned int loopy_propagate_until(Graph_t g, float convergence, unsigned int max_iterations)
+/* the drop-in convergence drivers: run on the device mirror, download */
+static unsigned int dropin_run(Graph_t g, const char *who, int mode, int batched, float convergence,
+                               unsigned int max_iterations)
 {
-    bp_dev *d = bp_dropin_device(g, "loopy_propagate_until");
+    bp_dev *d = bp_dropin_device(g, who);
     if (!d)
         return 0;
     unsigned int result = 0;
-    if (bp_loopy_until(d, convergence, max_iterations, &result) || bp_download(d, g)) {
-        bp_fail("loopy_propagate_until: %s", bp_last_error());
+    double tail[2] = {-1.0, 0.0};
+    int rc = hipSetDevice(d->device) == hipSuccess ? 0 : -1;
+    if (rc == 0)
+        rc = batched ? run_batched(d, mode, convergence, max_iterations, &result)
+                     : run_until(d, mode, convergence, max_iterations, &result, tail);
+    if (rc || bp_download(d, g)) {
+        bp_fail("%s: %s", who, bp_last_error());
         return 0;
     }
-    if (result == max_iterations)
-        printf("No Convergence: after %u iterations\n", max_iterations);
+    if (!batched && result == max_iterations)
+        printf("No Convergence: previous: %f vs current: %f\n", tail[0], tail[1]);
     return result;
 }
 
-extern "C" unsigned int loopy_propagate_until_hip(Graph_t g, float convergence, unsigned int max_iterations)
+extern "C" unsigned int loopy_propagate_until(Graph_t g, float convergence, unsigned int max_iterations)
+{
+    return dropin_run(g, "loopy_propagate_until", ITER, 0, convergence, max_iterations);
+}
+
+extern "C" unsigned int loopy_propagate_until_edge(Graph_t g, float convergence, unsigned int max_iterations)
+{
+    return dropin_run(g, "loopy_propagate_until_edge", EDGE, 0, convergence, max_iterations);
+}
+
+extern "C" unsigned int loopy_progagate_until_acc(Graph_t g, float convergence, unsigned int max_iterations)
+{
+    return dropin_run(g, "loopy_progagate_until_acc", ITER, 1, convergence, max_iterations);
+}
+
+extern "C" unsigned int loopy_progagate_until_edge_acc(Graph_t g, float convergence, unsigned int max_iterations)
+{
+    return dropin_run(g, "loopy_progagate_until_edge_acc", EDGE, 1, convergence, max_iterations);
+}
+
+extern "C" void loopy_propagate_edge_one_iteration(Graph_t g)
+{
+    bp_dev *d = bp_dropin_device(g, "loopy_propagate_edge_one_iteration");
+    if (!d)
+        return;
+    if (bp_edge_iterate(d, 1) || bp_download(d, g))
+        bp_fail("loopy_propagate_edge_one_iteration: %s", bp_last_error());
+}
+
+/* The CUDA driver's contract (belief-propagation.cu:529): the caller has run
+ * init_previous_edge; the graph is uploaded, iterated to convergence and
+ * downloaded in one call. */
+static unsigned int hip_until(Graph_t g, int mode, float convergence, unsigned int max_iterations)
 {
     bp_dev *d = nullptr;
     bp_clear_error();
-    if (bp_upload(g, -1, &d) != 0)
+    if (upload_impl(g, -1, &d, false) != 0)
         return 0xffffffffu;
     unsigned int result = 0;
-    int rc = bp_loopy_init(d);
-    if (rc == 0)
-        rc = bp_loopy_until(d, convergence, max_iterations, &result);
+    int rc = run_until(d, mode, convergence, max_iterations, &result, nullptr);
     if (rc == 0)
         rc = bp_download(d, g);
     bp_free(d);
     return rc == 0 ? result : 0xffffffffu;
+}
+
+extern "C" unsigned int loopy_propagate_until_hip(Graph_t g, float convergence, unsigned int max_iterations)
+{
+    return hip_until(g, ITER, convergence, max_iterations);
+}
+
+extern "C" unsigned int loopy_propagate_until_hip_edge(Graph_t g, float convergence, unsigned int max_iterations)
+{
+    return hip_until(g, EDGE, convergence, max_iterations);
 }
 
 /* graph.c:426-448 on the GPU: one edge, tables in MAX_STATES stride */

@@ -45,7 +45,13 @@ constexpr int WAVE = 64;
 constexpr int BLOCK = 256;
 constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
 
-enum Mode { ITER = 0, INIT = 1, PREP = 2 };
+/* ITER  one node-centric loopy iteration (graph.c:1121-1183)
+ * INIT  init_previous_edge (graph.c:725-751) + the first combine buffer
+ * PREP  the combine buffer from given states and messages
+ * EDGE  one edge-centric iteration (graph.c:1186-1227, defects fixed):
+ *       messages from the sources' beliefs, beliefs multiplied by every
+ *       in-message without NaN skip, divided by their sum when it is > 0 */
+enum Mode { ITER = 0, INIT = 1, PREP = 2, EDGE = 3 };
 
 struct LoopyArgs {
     const unsigned *in_off;      /* [V+1] */
@@ -53,7 +59,8 @@ struct LoopyArgs {
     const unsigned char *slot_xy;/* [E] x | y << 4 (mixed networks only) */
     const float *J;              /* [E][S*S] */
     const unsigned char *nvars;  /* [V] (mixed networks only) */
-    float *states;               /* [V][SP] */
+    float *states;               /* [V][SP] (EDGE: the beliefs before the iteration) */
+    float *states_out;           /* [V][SP] EDGE: the beliefs after it */
     const float *buf_in;         /* [V][SP] gather source (ITER) */
     float *buf_out;              /* [V][SP] */
     const float *msg_in;         /* [E][SP] old messages (ITER) / messages (PREP) */
@@ -135,7 +142,7 @@ __device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, 
         x = xy & 15u;
         y = xy >> 4;
     }
-    const float *src = (MODE == INIT) ? a.states : a.buf_in;
+    const float *src = (MODE == INIT || MODE == EDGE) ? a.states : a.buf_in;
     float b[S];
 #pragma unroll
     for (int j = 0; j < S; ++j)
@@ -143,8 +150,8 @@ __device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, 
     send<S, MIXED>(a.J + (size_t)s * S * S, b, x, y, m);
 }
 
-/* Slot phase: the message of slot s, written to msg_out (ITER, INIT) with
- * its |new - old| added to dacc (ITER). */
+/* Slot phase: the message of slot s, written to msg_out (ITER, INIT, EDGE)
+ * with its |new - old| added to dacc (ITER, EDGE). */
 template <int S, bool MIXED, int MODE>
 __device__ __forceinline__ void slot_message(const LoopyArgs &a, unsigned s, float *m, double &dacc)
 {
@@ -153,7 +160,7 @@ __device__ __forceinline__ void slot_message(const LoopyArgs &a, unsigned s, flo
     compute_message<S, MIXED, MODE>(a, s, m, x);
     if (MODE == PREP)
         return;
-    if (MODE == ITER) {
+    if (MODE == ITER || MODE == EDGE) {
         const float *old = a.msg_in + (size_t)s * SP;
 #pragma unroll
         for (int i = 0; i < S; ++i) {
@@ -185,6 +192,43 @@ __device__ __forceinline__ void combine(float *buf, const float *m, unsigned n)
     }
 }
 
+/* the product of the edge-centric schedule: every factor, NaN included
+ * (graph.c:1034-1043) */
+template <int S, bool MIXED>
+__device__ __forceinline__ void combine_all(float *buf, const float *m, unsigned n)
+{
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        if (!MIXED || (unsigned)k < n)
+            buf[k] = buf[k] * m[k];
+}
+
+template <int S, bool MIXED, int MODE>
+__device__ __forceinline__ void combine_mode(float *buf, const float *m, unsigned n)
+{
+    if (MODE == EDGE)
+        combine_all<S, MIXED>(buf, m, n);
+    else
+        combine<S, MIXED>(buf, m, n);
+}
+
+/* divided by the sum only when it is positive (graph.c:1046-1061) */
+template <int S, bool MIXED>
+__device__ __forceinline__ void normalise_positive(float *v, unsigned n)
+{
+    float sum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        if (!MIXED || (unsigned)k < n)
+            sum = sum + v[k];
+    if (sum > 0.0f) {
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            if (!MIXED || (unsigned)k < n)
+                v[k] = v[k] / sum;
+    }
+}
+
 template <int S, bool MIXED>
 __device__ __forceinline__ void normalise(float *v, unsigned n)
 {
@@ -205,6 +249,12 @@ template <int S, bool MIXED, int MODE>
 __device__ __forceinline__ void write_node(const LoopyArgs &a, unsigned v, const float *st, const float *b, unsigned n)
 {
     constexpr int SP = Stride<S>::value;
+    if (MODE == EDGE) { /* the new beliefs are the product */
+#pragma unroll
+        for (int k = 0; k < SP; ++k)
+            a.states_out[(size_t)v * SP + k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
+        return;
+    }
     if (MODE == ITER) {
 #pragma unroll
         for (int k = 0; k < S; ++k)
@@ -286,12 +336,12 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 for (int k = 0; k < S; ++k)
                     m[k] = old[r][k];
             } else {
-                const float *src = (MODE == INIT) ? a.states : a.buf_in;
+                const float *src = (MODE == INIT || MODE == EDGE) ? a.states : a.buf_in;
                 float b[S];
                 load_row<S>(src + (size_t)u[r] * SP, b);
                 unsigned x = xy[r] & 15u, y = xy[r] >> 4;
                 send<S, MIXED>(J[r], b, x, y, m);
-                if (MODE == ITER) {
+                if (MODE == ITER || MODE == EDGE) {
 #pragma unroll
                     for (int i = 0; i < S; ++i) {
                         if (!MIXED || (unsigned)i < x) {
@@ -346,8 +396,10 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
 #pragma unroll
                 for (int k = 0; k < S; ++k)
                     m[k] = lds[k * W + q];
-                combine<S, MIXED>(b, m, n);
+                combine_mode<S, MIXED, MODE>(b, m, n);
             }
+            if (MODE == EDGE)
+                normalise_positive<S, MIXED>(b, n);
             write_node<S, MIXED, MODE>(a, v, st[r], b, n);
         }
     }
@@ -383,7 +435,7 @@ __device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *l
 #pragma unroll
                 for (int k = 0; k < S; ++k)
                     m[k] = lds[k * WAVE + q];
-                combine<S, MIXED>(acc, m, n);
+                combine_mode<S, MIXED, MODE>(acc, m, n);
             }
         }
         wave_sync();
@@ -414,6 +466,8 @@ __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned
             acc[k] = st[k];
         walk_wide<S, MIXED, MODE>(a, sb, se, lds, lane, dacc, false, acc, n);
     }
+    if (MODE == EDGE)
+        normalise_positive<S, MIXED>(acc, n);
     if (lane == 0)
         write_node<S, MIXED, MODE>(a, v, st, acc, n);
 }
@@ -442,7 +496,7 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
         else
             chunk_wide<S, MIXED, MODE>(a, nb, sb, se, lds[wave], lane, dacc);
     }
-    if (MODE == ITER) {
+    if (MODE == ITER || MODE == EDGE) {
         double d = wave_sum(dacc);
         if (lane == 0)
             red[wave] = d;

@@ -49,17 +49,18 @@ int bp_set_abort_on_error(int enabled); /* returns the previous setting */
 /* Number of HIP devices visible (0 without a GPU); never fails. */
 int bp_device_count(void);
 
-/* GPU propagation of the reference's CUDA entry point: uploads g, runs
- * init_previous_edge and up to max_iterations loopy iterations with the
- * stop test of graph.c:1313 evaluated on an FP64 delta after every
- * iteration, downloads node_states and both message arrays (with the
- * pointer swaps the CPU path would have made) and returns the reference's
- * return value (index of the converged iteration, or max_iterations).
- * Returns 0xffffffff on failure. */
+/* GPU propagation with the contract of the reference's CUDA entry point:
+ * the caller has run init_previous_edge (as src/cuda_benchmark/
+ * belief-propagation.cu:1081 does); g is uploaded, iterated until the stop
+ * test of graph.c:1313 fires on an FP64 delta (checked on the device after
+ * every iteration) or max_iterations, and node_states and both message
+ * arrays are downloaded with the pointer swaps the CPU path would have
+ * made.  Returns what loopy_propagate_until returns (the index of the
+ * converged iteration, or max_iterations); 0xffffffff on failure. */
 unsigned int loopy_propagate_until_hip(Graph_t g, float convergence, unsigned int max_iterations);
 
-/* The same through the edge-centric schedule (one pass over the edges,
- * then one over the nodes, per iteration). */
+/* The same through the edge-centric schedule of graph.c:1186-1276 with its
+ * defects fixed (see graph.h). */
 unsigned int loopy_propagate_until_hip_edge(Graph_t g, float convergence, unsigned int max_iterations);
 
 /* ---- device-resident handle ---------------------------------------------- */
@@ -93,6 +94,13 @@ int bp_loopy_iterate(bp_dev *d, unsigned int n, double *deltas);
 /* Like loopy_propagate_until: runs until the stop test fires or
  * max_iterations; *result receives the reference's return value. */
 int bp_loopy_until(bp_dev *d, float convergence, unsigned int max_iterations, unsigned int *result);
+
+/* n edge-centric iterations (loopy_propagate_edge_one_iteration,
+ * graph.c:1186-1227, defects fixed). */
+int bp_edge_iterate(bp_dev *d, unsigned int n);
+
+/* Like loopy_propagate_until_edge (graph.c:1230-1276). */
+int bp_edge_until(bp_dev *d, float convergence, unsigned int max_iterations, unsigned int *result);
 
 /* Tree BP: start, up-sweep over levels 1..L-2, down-sweep over L-1..1,
  * marginalize.  The Graph_t given to bp_upload must have its levels set

@@ -35,6 +35,8 @@
 
 #include "../include/graph.h"
 
+#pragma GCC diagnostic ignored "-Wunused-function"
+
 #define R float
 #define FN(x) orc_f32_##x
 #include "bp_oracle_core.h"
@@ -104,6 +106,12 @@ unsigned int orc_loopy_until(Graph_t g, float convergence, unsigned int max_iter
     unsigned int it = orc_f32_loopy_until(&v, convergence, max_iterations, threads, deltas);
     f32_commit(g, &v);
     return it;
+}
+
+void orc_edge_iteration(Graph_t g)
+{
+    orc_f32_view v = f32_view(g);
+    orc_f32_edge_iteration(&v);
 }
 
 float orc_delta(Graph_t g)

@@ -204,6 +204,38 @@ static unsigned int FN(loopy_until)(FN(view) *g, R convergence, unsigned int max
     return i;
 }
 
+/* ---- edge-centric schedule ------------------------------------------------
+ * graph.c:1186-1276 with its defects fixed (it copies E*10 bytes instead of
+ * floats, runs `parallel` without `for`, and normalises node 0 only):
+ *   prev <- cur (all of it); cur[e] = send(states[src(e)], e) for every e;
+ *   states[dst(e)][k] *= cur[e][k] for every e in edge order (no NaN skip,
+ *   graph.c:1034-1043); every node divided by its sum when that is > 0
+ *   (graph.c:1046-1061).  No pointer swap. */
+static void FN(edge_iteration)(FN(view) *g)
+{
+    R *P = *g->prev, *Cm = *g->cur;
+    memcpy(P, Cm, sizeof(R) * (size_t)g->E * MAX_STATES);
+    for (unsigned int e = 0; e < g->E; ++e) {
+        unsigned int u = g->src[e];
+        FN(send)(g, g->states + (size_t)MAX_STATES * u, g->num_vars[u], e, Cm + (size_t)MAX_STATES * e);
+    }
+    for (unsigned int e = 0; e < g->E; ++e) {
+        unsigned int v = g->dst[e], x = g->xdim[e], y = g->ydim[e];
+        R *s = g->states + (size_t)MAX_STATES * v;
+        for (unsigned int k = 0; k < y; ++k)
+            s[k] = s[k] * ((k < x) ? Cm[(size_t)MAX_STATES * e + k] : (R)0.0);
+    }
+    for (unsigned int v = 0; v < g->V; ++v) {
+        R *s = g->states + (size_t)MAX_STATES * v;
+        R sum = (R)0.0;
+        for (unsigned int k = 0; k < g->num_vars[v]; ++k)
+            sum = sum + s[k];
+        if (sum > (R)0.0)
+            for (unsigned int k = 0; k < g->num_vars[v]; ++k)
+                s[k] = s[k] / sum;
+    }
+}
+
 /* ---- tree BP ---------------------------------------------------------- */
 
 static unsigned int FN(level_end)(const FN(view) *g, unsigned int level)

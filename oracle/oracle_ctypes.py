@@ -36,6 +36,7 @@ def load():
         "orc_loopy_iteration": (None, [vp]),
         "orc_loopy_iteration_omp": (None, [vp, C.c_int]),
         "orc_loopy_until": (C.c_uint, [vp, C.c_float, C.c_uint, C.c_int, C.POINTER(C.c_float)]),
+        "orc_edge_iteration": (None, [vp]),
         "orc_delta": (C.c_float, [vp]),
         "orc_delta64": (C.c_double, [vp]),
         "orc_send_message": (None, [C.POINTER(C.c_float), C.c_uint, vp, C.c_uint, C.POINTER(C.c_float)]),
@@ -80,6 +81,10 @@ def loopy_until(g, conv: float, max_iter: int, threads: int = 1):
     d = (C.c_float * max(max_iter, 1))()
     it = load().orc_loopy_until(_vp(g), conv, max_iter, threads, d)
     return it, np.array(d[: min(it + 1, max_iter)], dtype=np.float32)
+
+
+def edge_iteration(g):
+    load().orc_edge_iteration(_vp(g))
 
 
 def delta(g) -> float:

@@ -25,6 +25,7 @@
  *       tree [LEVELS]       levels + the driver's up/down sweeps + marginalize;
  *                           LEVELS (u32 num_levels, u32[2V]) replaces the
  *                           reference's O(V^2) level pass for large trees
+ *       edge N              init_previous_edge + N x loopy_propagate_edge_one_iteration
  *       time N REPS         wall-clock of init + N iterations (best of REPS)
  *
  * Built with -Dfloat=double (HARNESS_FP64) the graph core runs in double; only
@@ -286,6 +287,14 @@ int main(int argc, char **argv)
         put_padded("states", g->node_states, g->current_num_vertices, g->node_num_vars);
         dump_messages(g, "cur", "prev");
         free(deltas);
+    } else if (strcmp(action, "edge") == 0) {
+        /* the reference's edge-centric iteration, defects included */
+        unsigned int n = (unsigned int)atoi(argv[4]);
+        init_previous_edge(g);
+        for (unsigned int it = 0; it < n; ++it)
+            loopy_propagate_edge_one_iteration(g);
+        put_padded("states", g->node_states, g->current_num_vertices, g->node_num_vars);
+        put_padded("cur", *g->current_edge_messages, g->current_num_edges, g->edges_x_dim);
     } else if (strcmp(action, "until") == 0) {
         float conv = (float)atof(argv[4]);
         unsigned int max = (unsigned int)atoi(argv[5]);

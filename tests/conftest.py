@@ -91,6 +91,8 @@ def graph_from_structure(d: dict):
 def overrides(d: dict) -> dict:
     import ast
 
+    if "overrides" not in d:
+        return {}
     return dict(ast.literal_eval(str(d["overrides"])))
 
 

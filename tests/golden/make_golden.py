@@ -71,6 +71,11 @@ def struct_arrays(d: dict) -> dict:
     V, E = int(d["meta"][0]), int(d["meta"][1])
     S = int(d["node_num_vars"].max()) if V else 1
     out["node_states"] = compact(d["node_states"], V, S)
+    if "levels_to_nodes" in out:
+        # slots [num_levels, V) are never written by the reference (uninitialised)
+        lv = out["levels_to_nodes"].copy()
+        lv[int(d["num_levels"][0]):V] = 0
+        out["levels_to_nodes"] = lv
     out["edges_J"] = d["edges_J"].reshape(E, 10, 10)[:, :S, :S].copy()
     return out
 
@@ -92,6 +97,17 @@ def write_replay(g, path):
     J = B.view_f32(G.edges_joint_probabilities, E * 100).reshape(E, 10, 10)
     with open(path, "wb") as f:
         f.write(np.array([V, E], dtype=np.uint32).tobytes())
+        if V and (nv == nv[0]).all():
+            n = int(nv[0])
+            rec = np.zeros(V, dtype=[("n", "<u4"), ("obs", "u1"), ("st", "<f4", (n,)), ("name", "u1", (50,))])
+            rec["n"], rec["obs"], rec["st"], rec["name"] = nv, obs, st[:, :n], names
+            f.write(rec.tobytes())
+            x = n
+            erec = np.zeros(E, dtype=[("hdr", "<u4", (4,)), ("J", "<f4", (x * x,))])
+            erec["hdr"] = np.stack([src, dst, xs, ys], axis=1)
+            erec["J"] = J[:, :x, :x].reshape(E, x * x)
+            f.write(erec.tobytes())
+            return
         for v in range(V):
             n = int(nv[v])
             f.write(np.array([n], dtype=np.uint32).tobytes())
@@ -159,6 +175,11 @@ def small_case(name, kind, path, iters=20, until=(1e-3, 200), tree=True, fp64=No
         arrays["until_prev"] = compact(u["prev"], E, S)
         om = harness(kind, path, ["loopyq", str(iters)], omp=True, env={"OMP_NUM_THREADS": "4"})
         arrays["omp_states"] = compact(om["states"], V, S)
+        # the reference's edge-centric schedule (defective: only node 0 is
+        # normalised), a few iterations
+        ed = harness(kind, path, ["edge", "3"])
+        arrays["edge3_states"] = compact(ed["states"], V, S)
+        arrays["edge3_cur"] = compact(ed["cur"], E, S)
     if square:
         if tree:
             t = harness(kind, path, ["tree"])
@@ -201,23 +222,34 @@ def gen_case(name, preset, iters, until=(1e-3, 200), tree=True, full_struct=True
                full_struct=full_struct, loopy=loopy)
 
 
-def large_loopy(name, preset, iters, sample=2000, **kw):
-    """Digests of a full-size run: structure, final states/messages, deltas, a sample."""
+def large_loopy(name, preset, iters, sample=2000, via="xml", **kw):
+    """Digests of a full-size run: structure, final states/messages, deltas, a sample.
+
+    via="replay" rebuilds the graph through the reference's graph API instead
+    of its XMLBIF parser (whose libxml2 DOM and leaked XPath contexts do not
+    fit in memory at a million nodes); the parser itself is pinned by the
+    other cases."""
     print(name)
     bn, g, xml = generated_graph(preset, **kw)
     G = g.contents
     V, E = G.current_num_vertices, G.current_num_edges
-    d = harness("xml", xml, ["struct"])
+    src_kind, src_path = "xml", xml
+    if via == "replay":
+        src_kind, src_path = "replay", SCRATCH / f"{name}.replay"
+        write_replay(g, src_path)
+    d = harness(src_kind, src_path, ["struct"])
     arrays = struct_digest(d)
     arrays["xml_sha256"] = np.array(file_sha(xml))
     arrays["preset"] = np.array(preset)
+    arrays["overrides"] = np.array(repr(sorted(kw.items())))
+    arrays["via"] = np.array(via)
     arrays["meta"] = d["meta"]
     nv = d["node_num_vars"]
     S = int(nv.max())
     square = bool((nv == nv[0]).all())
     arrays["square"] = np.array(square)
     if square:
-        q = harness("xml", xml, ["loopyq", str(iters)])
+        q = harness(src_kind, src_path, ["loopyq", str(iters)])
         st = compact(q["states"], V, S)
         arrays["iters"] = np.array(iters)
         arrays["deltas"] = q["deltas"]
@@ -229,7 +261,8 @@ def large_loopy(name, preset, iters, sample=2000, **kw):
         arrays["sample_nodes"] = idx
         arrays["sample_states"] = st[idx]
         replay = SCRATCH / f"{name}.replay"
-        write_replay(g, replay)
+        if via != "replay":
+            write_replay(g, replay)
         q64 = harness("replay", replay, ["loopyq", str(iters)], fp64=True)
         st64 = compact(q64["states"], V, S)
         arrays["fp64_sha_states"] = np.array(sha(st64))
@@ -250,7 +283,9 @@ def large_tree(name, preset, sample=2000, **kw):
     V, E = G.current_num_vertices, G.current_num_edges
     lv = SCRATCH / f"{name}.levels"
     write_levels(g, lv)
-    t = harness("xml", xml, ["tree", str(lv)])
+    replay = SCRATCH / f"{name}.replay"
+    write_replay(g, replay)
+    t = harness("replay", replay, ["tree", str(lv)])
     S = 2
     st = compact(t["states"], V, S)
     arrays = {"xml_sha256": np.array(file_sha(xml)), "preset": np.array(preset),
@@ -320,7 +355,7 @@ CASES = {
     "bif": bif_cases,
     "c2b": lambda: large_loopy("c2b", "c2b", 20),
     "c2": lambda: large_loopy("c2", "c2", 20),
-    "c3": lambda: large_loopy("c3", "c3", 50),
+    "c3": lambda: large_loopy("c3", "c3", 50, via="replay"),
     "tree1m": lambda: large_tree("tree1m", "tree"),
 }
 

@@ -117,8 +117,8 @@ def test_until_bitwise(lib, case):
     conv, mx = float(d["until_conv"]), int(d["until_max"])
     for fn in ("dropin", "hip"):
         g = graph_for_case(d)
+        lib.init_previous_edge(g)
         if fn == "dropin":
-            lib.init_previous_edge(g)
             it = lib.loopy_propagate_until(g, conv, mx)
         else:
             it = lib.loopy_propagate_until_hip(g, conv, mx)
@@ -233,3 +233,78 @@ def test_upload_errors_are_reported(lib):
     assert lib.bp_upload(g, -1, C.byref(h)) == -1  # 2 of 3 nodes added
     assert b"nodes" in lib.bp_last_error()
     lib.graph_destroy(g)
+
+
+@pytest.mark.parametrize("case", ["example", "dog_xml", "c1", "lattice16", "c4net_small"])
+def test_edge_schedule_bitwise(lib, orc, case):
+    """Edge-centric iterations on the GPU == the oracle's, bit for bit,
+    through the handle and through loopy_propagate_edge_one_iteration."""
+    print(f"case {case}", flush=True)
+    d = golden(case)
+    g = graph_for_case(d)
+    h = graph_for_case(d)
+    k = graph_for_case(d)
+    orc.init_previous_edge(h)
+    for _ in range(5):
+        orc.edge_iteration(h)
+    dev = B.Device(g)
+    dev.init()
+    B.check(lib.bp_edge_iterate(dev.h, 5), "bp_edge_iterate")
+    dev.download(g)
+    for which in ("cur", "prev"):
+        assert same_bits(B.messages(g, which), B.messages(h, which)), which
+    assert same_bits(B.states(g), B.states(h)), bits_report(B.states(g), B.states(h))
+    dev.free()
+    lib.init_previous_edge(k)
+    for _ in range(5):
+        lib.loopy_propagate_edge_one_iteration(k)
+    assert same_bits(B.states(k), B.states(h))
+    assert same_bits(B.messages(k, "cur"), B.messages(h, "cur"))
+    for x in (g, h, k):
+        lib.graph_destroy(x)
+
+
+def _expected_until(orc, g, step, conv, max_iter, batched):
+    """Return value of the until drivers, from FP64 deltas of the oracle."""
+    prev = -1.0
+    i = 0
+    num_iter = 0
+    while i < max_iter:
+        n = 10 if batched else 1
+        for _ in range(n):
+            step(g)
+        delta = orc.delta64(g)
+        if delta < conv or abs(delta - prev) < conv:
+            return num_iter if batched else i
+        prev = delta
+        num_iter += n
+        i += n
+    return num_iter if batched else max_iter
+
+
+@pytest.mark.parametrize("variant", ["until_edge", "acc", "edge_acc", "hip", "hip_edge"])
+def test_convergence_drivers(lib, orc, variant):
+    """loopy_propagate_until_edge, loopy_progagate_until_acc / _edge_acc and
+    the HIP entry points: return value and results against the oracle run
+    for the same number of iterations."""
+    d = golden("c1")
+    conv, mx = 1e-3, 200
+    edge = variant in ("until_edge", "edge_acc", "hip_edge")
+    batched = variant in ("acc", "edge_acc")
+    h = graph_for_case(d)
+    orc.init_previous_edge(h)
+    step = orc.edge_iteration if edge else orc.loopy_iteration
+    want = _expected_until(orc, h, step, conv, mx, batched)
+    g = graph_for_case(d)
+    lib.init_previous_edge(g)
+    fn = {"until_edge": lib.loopy_propagate_until_edge, "acc": lib.loopy_progagate_until_acc,
+          "edge_acc": lib.loopy_progagate_until_edge_acc, "hip": lib.loopy_propagate_until_hip,
+          "hip_edge": lib.loopy_propagate_until_hip_edge}[variant]
+    got = fn(g, conv, mx)
+    assert got == want
+    # the oracle ran exactly as many iterations as the driver did
+    assert same_bits(B.states(g), B.states(h))
+    assert same_bits(B.messages(g, "cur"), B.messages(h, "cur"))
+    assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
+    lib.graph_destroy(g)
+    lib.graph_destroy(h)

@@ -262,3 +262,25 @@ def test_mixed_state_counts_zero_extension(lib, orc):
     lib.graph_destroy(g)
     lib.graph_destroy(h)
     lib.bp_bn_free(bn)
+
+
+@pytest.mark.parametrize("case", ["example", "dog_xml", "c1", "lattice16", "c4net_small"])
+def test_edge_schedule_against_reference(lib, orc, case):
+    """The edge-centric schedule with its defects fixed vs the reference's
+    (src/graph/graph.c:1186-1227), which normalises node 0 only: its beliefs
+    are the oracle's up to a per-node scale, so rows are compared after
+    normalising them (rounding differs by that rescaling: rtol 1e-5); the
+    messages, which are scale-free, match to the same tolerance."""
+    d = golden(case)
+    g = graph_for_case(d)
+    S = d["edge3_states"].shape[-1]
+    orc.init_previous_edge(g)
+    for _ in range(3):
+        orc.edge_iteration(g)
+    mine = states_of(g, S).astype(np.float64)
+    ref = d["edge3_states"].astype(np.float64)
+    sums = ref.sum(axis=1, keepdims=True)
+    ref_n = np.where(sums > 0, ref / np.where(sums > 0, sums, 1), ref)
+    np.testing.assert_allclose(mine, ref_n, rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(messages_of(g, "cur", S), d["edge3_cur"], rtol=1e-5, atol=1e-7)
+    lib.graph_destroy(g)
