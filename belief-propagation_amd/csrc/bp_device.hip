@@ -10,6 +10,7 @@
  * batch (fixed-iteration runs) or after every iteration with an on-device
  * stop flag (convergence runs), and one download.
  */
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,9 +27,10 @@
 
 using namespace bpk;
 
-/* Sums the per-block partials of `count` launches (fixed order, so the
- * result is reproducible).  until != 0: evaluates the stop test of
- * graph.c:1313 for launch `index` against *last and raises *stop. */
+/* Sums the per-block partials of `count` launches, one workgroup per
+ * launch (fixed order, so the result is reproducible).  until != 0 (count
+ * 1): evaluates the stop test of graph.c:1313 for launch `index` against
+ * *last and raises *stop. */
 __global__ __launch_bounds__(BLOCK) void k_finish(const double *partials, unsigned G, unsigned count,
                                                  double *deltas, double *last, unsigned *stop, float conv,
                                                  int until, unsigned index)
@@ -36,32 +38,39 @@ __global__ __launch_bounds__(BLOCK) void k_finish(const double *partials, unsign
     __shared__ double red[BLOCK];
     if (until && *stop)
         return;
-    for (unsigned it = 0; it < count; ++it) {
-        const double *p = partials + (size_t)it * G;
-        double acc = 0.0;
-        for (unsigned i = threadIdx.x; i < G; i += BLOCK)
-            acc += p[i];
-        red[threadIdx.x] = acc;
+    unsigned it = blockIdx.x;
+    if (it >= count)
+        return;
+    const double *p = partials + (size_t)it * G;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    unsigned i = threadIdx.x;
+    for (; i + 3 * BLOCK < G; i += 4 * BLOCK) {
+        acc[0] += p[i];
+        acc[1] += p[i + BLOCK];
+        acc[2] += p[i + 2 * BLOCK];
+        acc[3] += p[i + 3 * BLOCK];
+    }
+    for (; i < G; i += BLOCK)
+        acc[0] += p[i];
+    red[threadIdx.x] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    for (unsigned w = BLOCK / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
-        for (unsigned w = BLOCK / 2; w > 0; w >>= 1) {
-            if (threadIdx.x < w)
-                red[threadIdx.x] += red[threadIdx.x + w];
-            __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double d = red[0];
+        deltas[until ? index : it] = d;
+        if (until) {
+            double prev = *last;
+            double change = d - prev;
+            if (change < 0.0)
+                change = -change;
+            if (d < (double)conv || change < (double)conv)
+                *stop = index + 1;
+            *last = d;
         }
-        if (threadIdx.x == 0) {
-            double d = red[0];
-            deltas[until ? index : it] = d;
-            if (until) {
-                double prev = *last;
-                double change = d - prev;
-                if (change < 0.0)
-                    change = -change;
-                if (d < (double)conv || change < (double)conv)
-                    *stop = index + 1;
-                *last = d;
-            }
-        }
-        __syncthreads();
     }
 }
 
@@ -345,7 +354,7 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
         slot_xy[s] = (unsigned char)(g->edges_x_dim[e] | (g->edges_y_dim[e] << 4));
     }
     std::vector<uint2> chunks;
-    build_chunks(in_off, V, WAVE * (S <= 4 ? 2u : 1u), chunks);
+    build_chunks(in_off, V, WAVE * (S <= 4 ? (unsigned)BP_SLOTS_SMALL : 1u), chunks);
     d->nchunks = (unsigned)chunks.size() - 1;
     /* one chunk per wave; a multiple of 8 blocks so each XCD gets one range */
     unsigned want = (d->nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
@@ -428,43 +437,57 @@ static LoopyArgs make_args(bp_dev *d)
     return a;
 }
 
-template <int S, bool MIXED> static void launch_s(int mode, const LoopyArgs &a, unsigned G, hipStream_t st)
+/* ev (nullable) = {start, stop}: the launch is bracketed by events taken at
+ * the kernel's own begin and end (hipExtLaunchKernelGGL), so their elapsed
+ * time is the kernel's duration alone, without the gap between launches. */
+template <int S, bool MIXED, int MODE>
+static void launch_k(const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
 {
     dim3 grid(G), block(BLOCK);
-    if (mode == ITER)
-        hipLaunchKernelGGL((k_loopy<S, MIXED, ITER>), grid, block, 0, st, a);
-    else if (mode == INIT)
-        hipLaunchKernelGGL((k_loopy<S, MIXED, INIT>), grid, block, 0, st, a);
-    else if (mode == EDGE)
-        hipLaunchKernelGGL((k_loopy<S, MIXED, EDGE>), grid, block, 0, st, a);
+    if (ev)
+        hipExtLaunchKernelGGL((k_loopy<S, MIXED, MODE>), grid, block, 0, st, ev[0], ev[1], 0, a);
     else
-        hipLaunchKernelGGL((k_loopy<S, MIXED, PREP>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_loopy<S, MIXED, MODE>), grid, block, 0, st, a);
 }
 
-template <bool MIXED> static void launch_m(unsigned S, int mode, const LoopyArgs &a, unsigned G, hipStream_t st)
+template <int S, bool MIXED>
+static void launch_s(int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
+{
+    if (mode == ITER)
+        launch_k<S, MIXED, ITER>(a, G, st, ev);
+    else if (mode == INIT)
+        launch_k<S, MIXED, INIT>(a, G, st, ev);
+    else if (mode == EDGE)
+        launch_k<S, MIXED, EDGE>(a, G, st, ev);
+    else
+        launch_k<S, MIXED, PREP>(a, G, st, ev);
+}
+
+template <bool MIXED>
+static void launch_m(unsigned S, int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
 {
     switch (S) {
-    case 1: launch_s<1, MIXED>(mode, a, G, st); break;
-    case 2: launch_s<2, MIXED>(mode, a, G, st); break;
-    case 3: launch_s<3, MIXED>(mode, a, G, st); break;
-    case 4: launch_s<4, MIXED>(mode, a, G, st); break;
-    case 5: launch_s<5, MIXED>(mode, a, G, st); break;
-    case 6: launch_s<6, MIXED>(mode, a, G, st); break;
-    case 7: launch_s<7, MIXED>(mode, a, G, st); break;
-    case 8: launch_s<8, MIXED>(mode, a, G, st); break;
-    case 9: launch_s<9, MIXED>(mode, a, G, st); break;
-    default: launch_s<10, MIXED>(mode, a, G, st); break;
+    case 1: launch_s<1, MIXED>(mode, a, G, st, ev); break;
+    case 2: launch_s<2, MIXED>(mode, a, G, st, ev); break;
+    case 3: launch_s<3, MIXED>(mode, a, G, st, ev); break;
+    case 4: launch_s<4, MIXED>(mode, a, G, st, ev); break;
+    case 5: launch_s<5, MIXED>(mode, a, G, st, ev); break;
+    case 6: launch_s<6, MIXED>(mode, a, G, st, ev); break;
+    case 7: launch_s<7, MIXED>(mode, a, G, st, ev); break;
+    case 8: launch_s<8, MIXED>(mode, a, G, st, ev); break;
+    case 9: launch_s<9, MIXED>(mode, a, G, st, ev); break;
+    default: launch_s<10, MIXED>(mode, a, G, st, ev); break;
     }
 }
 
-static void launch(bp_dev *d, int mode, const LoopyArgs &a)
+static void launch(bp_dev *d, int mode, const LoopyArgs &a, const hipEvent_t *ev = nullptr)
 {
     if (d->V == 0)
         return;
     if (d->mixed)
-        launch_m<true>(d->S, mode, a, d->G, d->stream);
+        launch_m<true>(d->S, mode, a, d->G, d->stream, ev);
     else
-        launch_m<false>(d->S, mode, a, d->G, d->stream);
+        launch_m<false>(d->S, mode, a, d->G, d->stream, ev);
 }
 
 /* states (.) prod(previous messages) into buf[bcur] */
@@ -511,7 +534,7 @@ static void enqueue_iterations(bp_dev *d, unsigned n, int latest, int bcur)
         latest ^= 1;
         bcur ^= 1;
     }
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, d->stream, d->partials, d->G, n, d->deltas, d->last,
+    hipLaunchKernelGGL(k_finish, dim3(n), dim3(BLOCK), 0, d->stream, d->partials, d->G, n, d->deltas, d->last,
                        d->stop, 0.0f, 0, 0u);
 }
 
@@ -792,7 +815,7 @@ extern "C" int bp_profile_loopy(bp_dev *d, unsigned int n, float *ms)
     HIP_TRY(hipSetDevice(d->device));
     if (ensure_buffer(d))
         return -1;
-    std::vector<hipEvent_t> ev(n + 1);
+    std::vector<hipEvent_t> ev(2 * (size_t)n);
     for (auto &e : ev)
         HIP_TRY(hipEventCreate(&e));
     unsigned done = 0;
@@ -801,18 +824,16 @@ extern "C" int bp_profile_loopy(bp_dev *d, unsigned int n, float *ms)
         for (unsigned i = 0; i < batch; ++i) {
             LoopyArgs a = make_args(d);
             a.partials = d->partials + (size_t)i * d->G;
-            HIP_TRY(hipEventRecord(ev[done + i], d->stream));
-            launch(d, ITER, a);
+            launch(d, ITER, a, &ev[2 * (size_t)(done + i)]);
             d->latest ^= 1;
             d->bcur ^= 1;
             d->swaps++;
         }
-        HIP_TRY(hipEventRecord(ev[done + batch], d->stream));
-        hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, d->stream, d->partials, d->G, batch, d->deltas,
+        hipLaunchKernelGGL(k_finish, dim3(batch), dim3(BLOCK), 0, d->stream, d->partials, d->G, batch, d->deltas,
                            d->last, d->stop, 0.0f, 0, 0u);
         HIP_TRY(hipStreamSynchronize(d->stream));
         for (unsigned i = 0; i < batch; ++i)
-            HIP_TRY(hipEventElapsedTime(&ms[done + i], ev[done + i], ev[done + i + 1]));
+            HIP_TRY(hipEventElapsedTime(&ms[done + i], ev[2 * (size_t)(done + i)], ev[2 * (size_t)(done + i) + 1]));
         done += batch;
     }
     for (auto &e : ev)

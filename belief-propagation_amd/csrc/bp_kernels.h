@@ -39,6 +39,20 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+/* Compile-time tuning (measured choices are the defaults). */
+#ifndef BP_NT_LOADS
+#define BP_NT_LOADS 0
+#endif
+#ifndef BP_NT_STORES
+#define BP_NT_STORES 0
+#endif
+#ifndef BP_SLOTS_SMALL
+#define BP_SLOTS_SMALL 2
+#endif
+#ifndef BP_XCD_REMAP
+#define BP_XCD_REMAP 1
+#endif
+
 namespace bpk {
 
 constexpr int WAVE = 64;
@@ -75,6 +89,66 @@ template <int S> struct Stride {
     static constexpr int value = (S <= 2) ? S : (S <= 4) ? 4 : (S <= 8) ? 8 : 10;
 };
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+
+/* Streaming loads and stores: data read or written once per launch (slot
+ * arrays, tables, messages, a node's own belief) are marked non-temporal so
+ * they do not displace the combine buffers, which every XCD gathers at
+ * random, from L2. */
+template <int N> __device__ __forceinline__ void load_row(const float *__restrict__ p, float *out);
+
+template <int N>
+__device__ __forceinline__ void load_row_nt(const float *__restrict__ p, float *out)
+{
+    if constexpr (!BP_NT_LOADS) {
+        load_row<N>(p, out);
+    } else if constexpr (N % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i) {
+            v4f v = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(p) + i);
+            out[4 * i] = v.x;
+            out[4 * i + 1] = v.y;
+            out[4 * i + 2] = v.z;
+            out[4 * i + 3] = v.w;
+        }
+    } else if constexpr (N % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i) {
+            v2f v = __builtin_nontemporal_load(reinterpret_cast<const v2f *>(p) + i);
+            out[2 * i] = v.x;
+            out[2 * i + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            out[i] = __builtin_nontemporal_load(p + i);
+    }
+}
+
+template <int N>
+__device__ __forceinline__ void store_row_nt(float *__restrict__ p, const float *v)
+{
+    if constexpr (!BP_NT_STORES) {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            p[i] = v[i];
+    } else if constexpr (N % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 4; ++i)
+            __builtin_nontemporal_store(v4f{v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]},
+                                        reinterpret_cast<v4f *>(p) + i);
+    } else if constexpr (N % 2 == 0) {
+#pragma unroll
+        for (int i = 0; i < N / 2; ++i)
+            __builtin_nontemporal_store(v2f{v[2 * i], v[2 * i + 1]}, reinterpret_cast<v2f *>(p) + i);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            __builtin_nontemporal_store(v[i], p + i);
+    }
+}
+
 __device__ __forceinline__ void wave_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -86,7 +160,7 @@ __device__ __forceinline__ void wave_sync()
  * (workgroups are dealt to the 8 XCDs round-robin). */
 __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned G)
 {
-    if ((G & 7u) != 0u)
+    if (!BP_XCD_REMAP || (G & 7u) != 0u)
         return b;
     return (b & 7u) * (G >> 3) + (b >> 3);
 }
@@ -256,10 +330,18 @@ __device__ __forceinline__ void write_node(const LoopyArgs &a, unsigned v, const
         return;
     }
     if (MODE == ITER) {
+        if (MIXED) {
 #pragma unroll
-        for (int k = 0; k < S; ++k)
-            if (!MIXED || (unsigned)k < n)
-                a.states[(size_t)v * SP + k] = st[k];
+            for (int k = 0; k < S; ++k)
+                if ((unsigned)k < n)
+                    a.states[(size_t)v * SP + k] = st[k];
+        } else {
+            float row[SP];
+#pragma unroll
+            for (int k = 0; k < SP; ++k)
+                row[k] = (k < S) ? st[k] : 0.0f;
+            store_row_nt<SP>(a.states + (size_t)v * SP, row);
+        }
     }
 #pragma unroll
     for (int k = 0; k < SP; ++k)
@@ -310,18 +392,22 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
         unsigned s = sb + r * WAVE + lane;
         if (s < se) {
             if (MODE != PREP) {
-                u[r] = a.slot_src[s];
-                load_row<S * S>(a.J + (size_t)s * S * S, J[r]);
+                u[r] = BP_NT_LOADS ? __builtin_nontemporal_load(a.slot_src + s) : a.slot_src[s];
+                load_row_nt<S * S>(a.J + (size_t)s * S * S, J[r]);
                 xy[r] = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
             }
             if (MODE != INIT)
-                load_row<S>(a.msg_in + (size_t)s * SP, old[r]);
+                load_row_nt<S>(a.msg_in + (size_t)s * SP, old[r]);
         }
         unsigned v = nb + r * WAVE + lane;
         if (v < ne) {
-            q0[r] = a.in_off[v] - sb;
-            q1[r] = a.in_off[v + 1] - sb;
-            load_row<S>(a.states + (size_t)v * SP, st[r]);
+            q0[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v) : a.in_off[v]) - sb;
+            q1[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v + 1) : a.in_off[v + 1]) - sb;
+            /* INIT and EDGE gather beliefs: keep them cached there */
+            if (MODE == ITER || MODE == PREP)
+                load_row_nt<S>(a.states + (size_t)v * SP, st[r]);
+            else
+                load_row<S>(a.states + (size_t)v * SP, st[r]);
             nn[r] = MIXED ? a.nvars[v] : S;
         }
     }
@@ -352,10 +438,11 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                         }
                     }
                 }
-                float *dst = a.msg_out + (size_t)s * SP;
+                float row[SP];
 #pragma unroll
                 for (int i = 0; i < SP; ++i)
-                    dst[i] = (i < S) ? m[i] : 0.0f;
+                    row[i] = (i < S) ? m[i] : 0.0f;
+                store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
             }
 #pragma unroll
             for (int k = 0; k < S; ++k)
@@ -474,7 +561,7 @@ __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned
 
 /* Slots per lane of the narrow path: two for small state counts. */
 template <int S> struct SlotsPerLane {
-    static constexpr int value = (S <= 4) ? 2 : 1;
+    static constexpr int value = (S <= 4) ? BP_SLOTS_SMALL : 1;
 };
 
 template <int S, bool MIXED, int MODE>

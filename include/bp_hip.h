@@ -123,8 +123,9 @@ int bp_dev_info(const bp_dev *d, uint64_t *info);
  *   + per node 12*n + 4: belief read and write, combine-buffer write, offset */
 double bp_loopy_algorithmic_bytes(const bp_dev *d);
 
-/* Times each of n iterations with HIP events recorded on the handle's
- * stream around its kernel launch (no graph).  ms[i] gets iteration i. */
+/* Times each of n iterations with HIP events taken at the start and end of
+ * its kernel (hipExtLaunchKernelGGL, no graph): the kernel's duration alone.
+ * ms[i] gets iteration i. */
 int bp_profile_loopy(bp_dev *d, unsigned int n, float *ms);
 
 #ifdef __cplusplus
