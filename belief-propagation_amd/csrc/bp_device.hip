@@ -359,6 +359,12 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
     /* one chunk per wave; a multiple of 8 blocks so each XCD gets one range */
     unsigned want = (d->nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
     unsigned G = std::max(8u, (want + 7u) & ~7u);
+    if (BP_PERSIST > 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0)
+            G = std::min(G, std::max(8u, ((unsigned)cus * BP_PERSIST + 7u) & ~7u));
+    }
     d->G = G;
     /* delta partials of up to part_cap iterations are reduced per batch */
     d->part_cap = (unsigned)std::min<size_t>(256, std::max<size_t>(1, (64u << 20) / (8ull * G)));

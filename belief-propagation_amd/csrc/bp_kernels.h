@@ -52,6 +52,11 @@
 #ifndef BP_XCD_REMAP
 #define BP_XCD_REMAP 1
 #endif
+/* > 0: the iteration grid is capped at this many workgroups per CU and each
+ * wave walks its share of chunks in a loop (0: one chunk per wave). */
+#ifndef BP_PERSIST
+#define BP_PERSIST 0
+#endif
 
 namespace bpk {
 
@@ -573,15 +578,22 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
     __shared__ float lds[WAVES_PER_BLOCK][S * WAVE * R];
     __shared__ double red[WAVES_PER_BLOCK];
     unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
-    unsigned c = xcd_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wave;
+    /* wave w of the grid takes the contiguous chunk range
+     * [w * nchunks / W, (w + 1) * nchunks / W), W = waves in the grid; with
+     * the XCD remap each XCD's waves cover one contiguous range */
+    unsigned W = gridDim.x * WAVES_PER_BLOCK;
+    unsigned w = xcd_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wave;
+    unsigned c0 = (unsigned)(((unsigned long long)w * a.nchunks) / W);
+    unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / W);
     double dacc = 0.0;
-    if (c < a.nchunks) {
+    for (unsigned c = c0; c < c1; ++c) {
         uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
         unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
         if (se - sb <= (unsigned)(WAVE * R))
             chunk_narrow<S, MIXED, MODE, R>(a, nb, ne, sb, se, lds[wave], lane, dacc);
         else
             chunk_wide<S, MIXED, MODE>(a, nb, sb, se, lds[wave], lane, dacc);
+        wave_sync();
     }
     if (MODE == ITER || MODE == EDGE) {
         double d = wave_sum(dacc);
