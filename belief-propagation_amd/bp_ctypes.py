@@ -90,6 +90,25 @@ class GenParams(C.Structure):
     ]
 
 
+class BN(C.Structure):
+    """bp_bn of include/bp_model.h (a network as an XMLBIF file holds it)."""
+
+    _fields_ = [
+        ("num_vars", C.c_uint),
+        ("names", C.c_char_p),
+        ("num_states", u32p),
+        ("num_defs", C.c_uint),
+        ("def_for", u32p),
+        ("def_given_begin", C.POINTER(C.c_uint64)),
+        ("def_given", u32p),
+        ("def_table_begin", C.POINTER(C.c_uint64)),
+        ("def_table", f32p),
+        ("given_cap", C.c_uint64),
+        ("table_cap", C.c_uint64),
+        ("def_cap", C.c_uint64),
+    ]
+
+
 _SIGNATURES = {
     # graph.h
     "create_graph": (GraphP, [C.c_uint, C.c_uint]),
@@ -170,6 +189,19 @@ _SIGNATURES = {
     "bp_bn_to_graph": (GraphP, [C.c_void_p]),
     "bp_bn_write_xmlbif": (C.c_int, [C.c_void_p, C.c_char_p]),
     "bp_bn_free": (None, [C.c_void_p]),
+    "bp_bn_create": (C.c_void_p, [C.c_uint]),
+    "bp_bn_set_var": (C.c_int, [C.c_void_p, C.c_uint, C.c_char_p, C.c_uint]),
+    "bp_bn_add_def": (C.c_int, [C.c_void_p, C.c_uint, u32p, C.c_uint, f32p, C.c_uint64]),
+    # bp_diff.h
+    "bp_diff_create": (C.c_int, [C.c_void_p, C.c_uint, C.c_int, C.POINTER(C.c_void_p)]),
+    "bp_diff_free": (None, [C.c_void_p]),
+    "bp_diff_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "bp_diff_rows": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), u32p, u32p]),
+    "bp_diff_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bp_diff_forward": (C.c_int, [C.c_void_p, C.c_uint, C.c_uint, C.c_void_p, C.c_void_p]),
+    "bp_diff_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bp_diff_backward_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bp_diff_tables": (C.c_int, [C.c_void_p, f32p]),
 }
 
 _lib = None
@@ -292,6 +324,22 @@ def generate(preset: str, **overrides):
     if not bn:
         raise RuntimeError(f"generator failed: {last_error()}")
     return bn
+
+
+def bn_arrays(bn) -> dict:
+    """Copies of a bp_bn's arrays (state counts, definitions, tables)."""
+    b = C.cast(bn, C.POINTER(BN)).contents
+    nd = b.num_defs
+    gb = np.ctypeslib.as_array(b.def_given_begin, shape=(nd + 1,)).copy()
+    tb = np.ctypeslib.as_array(b.def_table_begin, shape=(nd + 1,)).copy()
+    return {
+        "num_states": view_u32(b.num_states, b.num_vars).copy(),
+        "def_for": view_u32(b.def_for, nd).copy(),
+        "given_begin": gb,
+        "given": view_u32(b.def_given, int(gb[-1])).copy() if gb[-1] else np.zeros(0, np.uint32),
+        "table_begin": tb,
+        "table": view_f32(b.def_table, int(tb[-1])).copy() if tb[-1] else np.zeros(0, np.float32),
+    }
 
 
 def bn_to_graph(bn, csr: bool = True, levels: bool = False) -> GraphP:

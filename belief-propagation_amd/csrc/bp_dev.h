@@ -85,6 +85,13 @@ template <typename T> static int dmalloc(T **p, size_t count)
     return 0;
 }
 
+/* Loopy kernel launch shared with the adjoint (bp_diff.hip): the handle's
+ * arrays as kernel arguments, and one launch of k_loopy<S, MIXED, mode> on
+ * stream st over the handle's chunks (ev: optional start/stop events). */
+bpk::LoopyArgs bp_make_args(bp_dev *d);
+void bp_launch_loopy(bp_dev *d, int mode, const bpk::LoopyArgs &a, hipStream_t st, const hipEvent_t *ev = nullptr);
+int bp_upload_graph(Graph_t g, int device, bp_dev **out);
+
 /* tree BP (bp_tree.hip) */
 int bp_tree_plan_build(bp_dev *d, Graph_t g);
 void bp_tree_plan_free(bp_tree_plan *p);

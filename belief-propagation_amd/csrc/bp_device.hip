@@ -408,6 +408,9 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
 
 extern "C" int bp_upload(Graph_t g, int device, bp_dev **out) { return upload_impl(g, device, out, true); }
 
+/* an upload without the tree schedule, for the adjoint and batched handles */
+int bp_upload_graph(Graph_t g, int device, bp_dev **out) { return upload_impl(g, device, out, false); }
+
 extern "C" int bp_upload_state(bp_dev *d, Graph_t g)
 {
     bp_clear_error();
@@ -422,7 +425,7 @@ extern "C" int bp_upload_state(bp_dev *d, Graph_t g)
 /* ------------------------------------------------------------------------ */
 /* launches                                                                  */
 
-static LoopyArgs make_args(bp_dev *d)
+LoopyArgs bp_make_args(bp_dev *d)
 {
     LoopyArgs a;
     a.in_off = d->in_off;
@@ -465,6 +468,10 @@ static void launch_s(int mode, const LoopyArgs &a, unsigned G, hipStream_t st, c
         launch_k<S, MIXED, INIT>(a, G, st, ev);
     else if (mode == EDGE)
         launch_k<S, MIXED, EDGE>(a, G, st, ev);
+    else if (mode == FWD)
+        launch_k<S, MIXED, FWD>(a, G, st, ev);
+    else if (mode == FINIT)
+        launch_k<S, MIXED, FINIT>(a, G, st, ev);
     else
         launch_k<S, MIXED, PREP>(a, G, st, ev);
 }
@@ -486,14 +493,24 @@ static void launch_m(unsigned S, int mode, const LoopyArgs &a, unsigned G, hipSt
     }
 }
 
-static void launch(bp_dev *d, int mode, const LoopyArgs &a, const hipEvent_t *ev = nullptr)
+static void launch_on(bp_dev *d, int mode, const LoopyArgs &a, hipStream_t st, const hipEvent_t *ev = nullptr)
 {
     if (d->V == 0)
         return;
     if (d->mixed)
-        launch_m<true>(d->S, mode, a, d->G, d->stream, ev);
+        launch_m<true>(d->S, mode, a, d->G, st, ev);
     else
-        launch_m<false>(d->S, mode, a, d->G, d->stream, ev);
+        launch_m<false>(d->S, mode, a, d->G, st, ev);
+}
+
+static void launch(bp_dev *d, int mode, const LoopyArgs &a, const hipEvent_t *ev = nullptr)
+{
+    launch_on(d, mode, a, d->stream, ev);
+}
+
+void bp_launch_loopy(bp_dev *d, int mode, const LoopyArgs &a, hipStream_t st, const hipEvent_t *ev)
+{
+    launch_on(d, mode, a, st, ev);
 }
 
 /* states (.) prod(previous messages) into buf[bcur] */
@@ -501,7 +518,7 @@ static int ensure_buffer(bp_dev *d)
 {
     if (d->buf_valid)
         return 0;
-    LoopyArgs a = make_args(d);
+    LoopyArgs a = bp_make_args(d);
     a.buf_out = d->buf[d->bcur];
     launch(d, PREP, a);
     HIP_TRY(hipGetLastError());
@@ -517,7 +534,7 @@ extern "C" int bp_loopy_init(bp_dev *d)
         return -1;
     }
     HIP_TRY(hipSetDevice(d->device));
-    LoopyArgs a = make_args(d);
+    LoopyArgs a = bp_make_args(d);
     a.msg_out = d->msg[d->latest]; /* the previous-message array, graph.c:734 */
     a.buf_out = d->buf[d->bcur];
     launch(d, INIT, a);
@@ -530,7 +547,7 @@ extern "C" int bp_loopy_init(bp_dev *d)
 static void enqueue_iterations(bp_dev *d, unsigned n, int latest, int bcur)
 {
     for (unsigned i = 0; i < n; ++i) {
-        LoopyArgs a = make_args(d);
+        LoopyArgs a = bp_make_args(d);
         a.msg_in = d->msg[latest];
         a.msg_out = d->msg[latest ^ 1];
         a.buf_in = d->buf[bcur];
@@ -601,7 +618,7 @@ struct Roles {
 
 static LoopyArgs step_args(bp_dev *d, int mode, Roles &r)
 {
-    LoopyArgs a = make_args(d);
+    LoopyArgs a = bp_make_args(d);
     if (mode == ITER) {
         a.msg_in = d->msg[r.latest];
         a.msg_out = d->msg[r.latest ^ 1];
@@ -828,7 +845,7 @@ extern "C" int bp_profile_loopy(bp_dev *d, unsigned int n, float *ms)
     while (done < n) {
         unsigned batch = std::min(n - done, d->part_cap);
         for (unsigned i = 0; i < batch; ++i) {
-            LoopyArgs a = make_args(d);
+            LoopyArgs a = bp_make_args(d);
             a.partials = d->partials + (size_t)i * d->G;
             launch(d, ITER, a, &ev[2 * (size_t)(done + i)]);
             d->latest ^= 1;

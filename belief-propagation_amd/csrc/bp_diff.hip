@@ -1,0 +1,965 @@
+/*
+ * bp_diff.hip -- differentiable loopy BP: the parameter -> table map, the
+ * unrolled forward with checkpoints, and the hand-written adjoint.
+ *
+ * Parameters are the TABLEs of a network's CPT definitions.  k_tables
+ * rebuilds every slot's pairwise table from them with the XMLBIF reader's
+ * loop (src/bnf-xml-parser/xml-expression.c:384-400, restated in
+ * bp_model.c): for parent state j and child state k, the sequential f32 sum
+ * of the TABLE entries in the blocks of that parent's state, so the tables
+ * are bit-identical to the reader's.  k_initial puts root priors into the
+ * initial beliefs, as graph_set_node_state does (xml-expression.c:290-317).
+ *
+ * The forward is the loopy kernel in its FWD/FINIT modes: the node-centric
+ * iteration of src/graph/graph.c:1121-1183 from init_previous_edge
+ * (graph.c:725-751), writing every iterate to the buffer it is given.
+ *
+ * Adjoint of one iteration t (k_adjoint, one launch, same chunks as the
+ * forward).  In exact arithmetic, with N(x) = x / sum(x):
+ *   m_t[s]  = N(z_s), z_s = J_s . buf_{t-1}[u],  u = src(s)
+ *   T_t[v]  = prod_{s in in(v)} m_t[s]
+ *   st_t[v] = N(a_v), a_v = st_{t-1}[v] (.) T_t[v]   (a_v = st_{t-1}[v] without in-edges)
+ *   buf_t[v]= st_t[v] (.) T_t[v]
+ * Given dst_t (from the loss or iteration t+1) and dbuf_t[v] =
+ * sum_{s: src(s)=v} w_{t+1}[s]:
+ *   dst_t   += dbuf_t (.) T_t          dT = dbuf_t (.) st_t
+ *   da       = (dst_t - <dst_t, st_t>) / sum(a)   (identity when sum(a) <= 0,
+ *              where the forward divides by 1)
+ *   dst_{t-1} = da (.) T_t,  dT += da (.) st_{t-1}
+ *   dm_s     = dT (.) prod_{s' in in(v), s' != s} m_t[s']
+ *   dz_s     = (dm_s - <dm_s, m_s>) / sum(z_s)
+ *   dJ_s    += dz_s (x) buf_{t-1}[u],  w_t[s] = J_s^T dz_s
+ * Step 0 (init) has m_0 from st_0 and buf_0 = st_0 (.) T_0; its w_0 and
+ * dbuf_0 (.) T_0 add to dst_0.  Sums over a node's out-edges are gathers in
+ * slot order through a source-grouped slot list, so results do not depend
+ * on scheduling (no atomics).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "bp_dev.h"
+#include "../../include/bp_diff.h"
+
+using namespace bpk;
+
+namespace {
+
+constexpr unsigned NONE = 0xffffffffu;
+
+/* table of one slot: the parameters of its definition and the parent */
+struct JMap {
+    unsigned tb;     /* first parameter of the definition */
+    unsigned slice;  /* TABLE entries per child state */
+    unsigned offset; /* product of the state counts of the later GIVENs */
+    unsigned meta;   /* parent states | child states << 8 | reverse << 16 */
+};
+
+struct ParentRec {
+    unsigned offset, n_par, slot_f, slot_r;
+};
+
+struct AdjArgs {
+    const unsigned *in_off, *slot_src, *out_off, *out_slot;
+    const float *J;
+    const uint2 *chunk;
+    unsigned nchunks;
+    const float *st_prev; /* st_{t-1} */
+    const float *st_cur;  /* st_t (step 0: st_0) */
+    const float *bsrc;    /* buf_{t-1} (step 0: st_0) */
+    const float *gst_in;  /* dL/dst_t along paths through later iterates and the loss */
+    float *gst_out;       /* dL/dst_{t-1} through st_t (step 0: dL/dst_0 without w_0) */
+    const float *w_in;    /* w_{t+1} or NULL */
+    float *w_out;         /* w_t */
+    float *jbar;          /* [E][S*S] */
+    int jbar_first;       /* write instead of accumulate */
+};
+
+__global__ void k_copy(const float *__restrict__ src, float *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = src[i];
+}
+
+__global__ void k_tables(const float *__restrict__ P, const JMap *__restrict__ map, float *__restrict__ J,
+                         unsigned E, unsigned S)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t SS = (size_t)S * S;
+    if (t >= (size_t)E * SS)
+        return;
+    unsigned s = (unsigned)(t / SS), ij = (unsigned)(t % SS), i = ij / S, j = ij % S;
+    JMap m = map[s];
+    unsigned npar = m.meta & 255u, nch = (m.meta >> 8) & 255u, rev = m.meta >> 16;
+    unsigned jp = rev ? j : i, k = rev ? i : j;
+    if (jp >= npar || k >= nch) {
+        J[t] = 0.0f;
+        return;
+    }
+    unsigned long long offset = m.offset, stride = (unsigned long long)npar * offset, slice = m.slice;
+    const float *T = P + m.tb + (unsigned long long)k * slice;
+    float acc = 0.0f;
+    for (unsigned long long b = 0;; ++b) {
+        unsigned long long start = jp * offset + b * stride;
+        if (b == 0 ? start > slice : start + offset > slice)
+            break;
+        for (unsigned long long idx = start; idx < start + offset; ++idx)
+            acc += T[idx];
+    }
+    J[t] = acc;
+}
+
+__global__ void k_initial(const float *__restrict__ P, const unsigned *__restrict__ prior_tb,
+                          const float *__restrict__ base, float *__restrict__ st0, unsigned V, unsigned S, unsigned SP)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)V * SP)
+        return;
+    unsigned v = (unsigned)(t / SP), k = (unsigned)(t % SP);
+    unsigned tb = prior_tb[v];
+    st0[t] = (tb != NONE && k < S) ? P[tb + k] : base[t];
+}
+
+template <int S, int STEP0>
+__global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
+{
+    constexpr int SP = Stride<S>::value;
+    constexpr int R = SlotsPerLane<S>::value;
+    constexpr int W = WAVE * R;
+    __shared__ float lm[WAVES_PER_BLOCK][S * W];
+    __shared__ float ldt[WAVES_PER_BLOCK][S * W];
+    __shared__ unsigned short lnode[WAVES_PER_BLOCK][W];
+    __shared__ unsigned short lq[WAVES_PER_BLOCK][2 * W];
+    unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
+    unsigned Wt = gridDim.x * WAVES_PER_BLOCK;
+    unsigned w = xcd_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wave;
+    unsigned c0 = (unsigned)(((unsigned long long)w * a.nchunks) / Wt);
+    unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / Wt);
+    float *LM = lm[wave], *LDT = ldt[wave];
+    unsigned short *LN = lnode[wave], *LQ = lq[wave];
+    for (unsigned c = c0; c < c1; ++c) {
+        uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
+        unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
+        /* slot phase 1: the message of every slot, kept with its operands */
+        float Jr[R][S * S], b[R][S], m[R][S], sig[R];
+        bool pos[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            unsigned s = sb + r * WAVE + lane;
+            if (s < se) {
+                unsigned u = a.slot_src[s];
+                load_row<S * S>(a.J + (size_t)s * S * S, Jr[r]);
+                load_row<S>(a.bsrc + (size_t)u * SP, b[r]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            unsigned s = sb + r * WAVE + lane;
+            if (s < se) {
+                float sum = 0.0f;
+#pragma unroll
+                for (int i = 0; i < S; ++i) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < S; ++j)
+                        acc = acc + Jr[r][i * S + j] * b[r][j];
+                    m[r][i] = acc;
+                    sum = sum + acc;
+                }
+                pos[r] = sum > 0.0f;
+                if (!pos[r])
+                    sum = 1.0f;
+                sig[r] = sum;
+#pragma unroll
+                for (int i = 0; i < S; ++i) {
+                    m[r][i] = m[r][i] / sum;
+                    LM[i * W + r * WAVE + lane] = m[r][i];
+                }
+            }
+        }
+        wave_sync();
+        /* node phase: dT, dst_{t-1} */
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            unsigned v = nb + r * WAVE + lane;
+            if (v < ne) {
+                unsigned q0 = a.in_off[v] - sb, q1 = a.in_off[v + 1] - sb;
+                unsigned o0 = a.out_off[v], o1 = a.out_off[v + 1];
+                float st[S], g[S], T[S], gb[S], dT[S];
+                load_row<S>(a.st_cur + (size_t)v * SP, st);
+                load_row<S>(a.gst_in + (size_t)v * SP, g);
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    T[k] = 1.0f;
+                    gb[k] = 0.0f;
+                }
+                for (unsigned q = q0; q < q1; ++q) {
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        float mk = LM[k * W + q];
+                        if (mk == mk)
+                            T[k] = T[k] * mk;
+                    }
+                }
+                if (a.w_in) {
+                    for (unsigned o = o0; o < o1; ++o) {
+                        float x[S];
+                        load_row<S>(a.w_in + (size_t)a.out_slot[o] * SP, x);
+#pragma unroll
+                        for (int k = 0; k < S; ++k)
+                            gb[k] = gb[k] + x[k];
+                    }
+                }
+                float out[S];
+                if (STEP0) {
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        dT[k] = gb[k] * st[k];
+                        out[k] = g[k] + gb[k] * T[k];
+                    }
+                } else {
+                    float sp[S], ds[S], av[S];
+                    load_row<S>(a.st_prev + (size_t)v * SP, sp);
+                    bool indeg = q1 > q0;
+                    float sa = 0.0f, dot = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        ds[k] = g[k] + gb[k] * T[k];
+                        dT[k] = gb[k] * st[k];
+                        av[k] = indeg ? sp[k] * T[k] : sp[k];
+                        sa = sa + av[k];
+                        dot = dot + ds[k] * st[k];
+                    }
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        float da = (sa > 0.0f) ? (ds[k] - dot) / sa : ds[k];
+                        out[k] = indeg ? da * T[k] : da;
+                        if (indeg)
+                            dT[k] = dT[k] + da * sp[k];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < SP; ++k)
+                    a.gst_out[(size_t)v * SP + k] = (k < S) ? out[k] : 0.0f;
+                unsigned l = r * WAVE + lane;
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    LDT[k * W + l] = dT[k];
+                LQ[2 * l] = (unsigned short)q0;
+                LQ[2 * l + 1] = (unsigned short)q1;
+                for (unsigned q = q0; q < q1; ++q)
+                    LN[q] = (unsigned short)l;
+            }
+        }
+        wave_sync();
+        /* slot phase 2: dm, dz, dJ, w */
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            unsigned s = sb + r * WAVE + lane;
+            if (s < se) {
+                unsigned q = r * WAVE + lane;
+                unsigned l = LN[q], q0 = LQ[2 * l], q1 = LQ[2 * l + 1];
+                float P[S];
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    P[k] = 1.0f;
+                for (unsigned p = q0; p < q1; ++p) {
+                    if (p == q)
+                        continue;
+#pragma unroll
+                    for (int k = 0; k < S; ++k) {
+                        float mk = LM[k * W + p];
+                        if (mk == mk)
+                            P[k] = P[k] * mk;
+                    }
+                }
+                float dm[S], dz[S], dot = 0.0f;
+#pragma unroll
+                for (int k = 0; k < S; ++k) {
+                    dm[k] = (m[r][k] == m[r][k]) ? LDT[k * W + l] * P[k] : 0.0f;
+                    dot = dot + dm[k] * m[r][k];
+                }
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    dz[k] = pos[r] ? (dm[k] - dot) / sig[r] : dm[k];
+                float *jb = a.jbar + (size_t)s * S * S;
+                float gJ[S * S];
+                if (!a.jbar_first)
+                    load_row<S * S>(jb, gJ);
+#pragma unroll
+                for (int i = 0; i < S; ++i)
+#pragma unroll
+                    for (int j = 0; j < S; ++j)
+                        gJ[i * S + j] = a.jbar_first ? dz[i] * b[r][j] : gJ[i * S + j] + dz[i] * b[r][j];
+#pragma unroll
+                for (int i = 0; i < S * S; ++i)
+                    jb[i] = gJ[i];
+                float wv[SP];
+#pragma unroll
+                for (int j = 0; j < SP; ++j) {
+                    float acc = 0.0f;
+                    if (j < S) {
+#pragma unroll
+                        for (int i = 0; i < S; ++i)
+                            acc = acc + Jr[r][i * S + j] * dz[i];
+                    }
+                    wv[j] = acc;
+                }
+#pragma unroll
+                for (int j = 0; j < SP; ++j)
+                    a.w_out[(size_t)s * SP + j] = wv[j];
+            }
+        }
+        wave_sync();
+    }
+}
+
+/* dL/dst_0 = the step-0 partial + sum of w_0 over the node's out-slots */
+__global__ void k_initial_grad(const float *__restrict__ part, const float *__restrict__ w0,
+                               const unsigned *__restrict__ out_off, const unsigned *__restrict__ out_slot,
+                               float *__restrict__ out, unsigned V, unsigned S, unsigned SP)
+{
+    unsigned v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V)
+        return;
+    for (unsigned k = 0; k < SP; ++k) {
+        float acc = part[(size_t)v * SP + k];
+        if (k < S)
+            for (unsigned o = out_off[v]; o < out_off[v + 1]; ++o)
+                acc = acc + w0[(size_t)out_slot[o] * SP + k];
+        out[(size_t)v * SP + k] = (k < S) ? acc : 0.0f;
+    }
+}
+
+/* dL/dparam: the transpose of k_tables and k_initial */
+__global__ void k_param_grad(const float *__restrict__ jbar, const float *__restrict__ dst0,
+                             const unsigned *__restrict__ def_of, const unsigned *__restrict__ def_tb,
+                             const unsigned *__restrict__ def_slice, const unsigned *__restrict__ def_pbegin,
+                             const ParentRec *__restrict__ parents, const unsigned *__restrict__ def_prior,
+                             float *__restrict__ out, unsigned num_params, unsigned S, unsigned SP)
+{
+    unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= num_params)
+        return;
+    unsigned d = def_of[p];
+    unsigned local = p - def_tb[d], slice = def_slice[d];
+    unsigned k = local / slice, idx = local % slice;
+    float g = 0.0f;
+    for (unsigned i = def_pbegin[d]; i < def_pbegin[d + 1]; ++i) {
+        ParentRec pr = parents[i];
+        unsigned j = (idx / pr.offset) % pr.n_par;
+        g = g + jbar[(size_t)pr.slot_f * S * S + j * S + k];
+        if (pr.slot_r != NONE)
+            g = g + jbar[(size_t)pr.slot_r * S * S + k * S + j];
+    }
+    unsigned v = def_prior[d];
+    if (v != NONE)
+        g = g + dst0[(size_t)v * SP + k];
+    out[p] = g;
+}
+
+/* slot order -> reference edge order */
+__global__ void k_slot_to_edge(const float *__restrict__ src, const unsigned *__restrict__ slot_edge,
+                               float *__restrict__ dst, unsigned E, unsigned width)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)E * width)
+        return;
+    unsigned s = (unsigned)(t / width), i = (unsigned)(t % width);
+    dst[(size_t)slot_edge[s] * width + i] = src[t];
+}
+
+inline unsigned blocks_for(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+} // namespace
+
+struct bp_diff {
+    bp_dev *d = nullptr;
+    unsigned V = 0, E = 0, S = 0, SP = 0;
+    unsigned num_params = 0, num_rows = 0;
+    std::vector<uint64_t> row_first;
+    std::vector<uint32_t> row_stride, row_len;
+
+    JMap *jmap = nullptr;
+    unsigned *prior_tb = nullptr;
+    unsigned *out_off = nullptr, *out_slot = nullptr, *slot_edge = nullptr;
+    unsigned *def_of = nullptr, *def_tb = nullptr, *def_slice = nullptr, *def_pbegin = nullptr, *def_prior = nullptr;
+    ParentRec *parents = nullptr;
+
+    float *P = nullptr;     /* parameters */
+    float *st0 = nullptr;   /* initial beliefs */
+    float *gP = nullptr;    /* dL/dparams */
+    float *gin = nullptr;   /* dL/dbeliefs (copy) */
+    float *gst[2] = {nullptr, nullptr};
+    float *w[2] = {nullptr, nullptr};
+    float *jbar = nullptr;
+    float *dst0p = nullptr, *dst0 = nullptr;
+
+    /* iterates: checkpoints t = 0, c, 2c, ...; a segment's recomputed ones;
+     * the forward's scratch; the last beliefs */
+    unsigned K = 0, c = 0, nck = 0;
+    bool have_forward = false, have_params = false;
+    float *ck_st = nullptr, *ck_buf = nullptr;
+    float *seg_st = nullptr, *seg_buf = nullptr;
+    float *fw_st[2] = {nullptr, nullptr}, *fw_buf[2] = {nullptr, nullptr};
+    float *final_st = nullptr;
+    std::map<std::pair<unsigned, unsigned>, hipGraphExec_t> fwd_graphs, bwd_graphs;
+};
+
+static void diff_release_storage(bp_diff *x)
+{
+    float *ps[] = {x->ck_st, x->ck_buf, x->seg_st, x->seg_buf, x->fw_st[0], x->fw_st[1], x->fw_buf[0], x->fw_buf[1],
+                   x->final_st};
+    for (float *p : ps)
+        if (p)
+            (void)hipFree(p);
+    x->ck_st = x->ck_buf = x->seg_st = x->seg_buf = x->final_st = nullptr;
+    x->fw_st[0] = x->fw_st[1] = x->fw_buf[0] = x->fw_buf[1] = nullptr;
+    for (auto &kv : x->fwd_graphs)
+        (void)hipGraphExecDestroy(kv.second);
+    for (auto &kv : x->bwd_graphs)
+        (void)hipGraphExecDestroy(kv.second);
+    x->fwd_graphs.clear();
+    x->bwd_graphs.clear();
+    x->K = x->c = x->nck = 0;
+}
+
+extern "C" void bp_diff_free(bp_diff *x)
+{
+    if (!x)
+        return;
+    if (x->d) {
+        (void)hipSetDevice(x->d->device);
+        (void)hipDeviceSynchronize();
+    }
+    diff_release_storage(x);
+    void *ps[] = {x->jmap, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
+                  x->def_pbegin, x->def_prior, x->parents, x->P, x->st0, x->gP, x->gin, x->gst[0], x->gst[1],
+                  x->w[0], x->w[1], x->jbar, x->dst0p, x->dst0};
+    for (void *p : ps)
+        if (p)
+            (void)hipFree(p);
+    bp_free(x->d);
+    delete x;
+}
+
+template <typename T> static int upload_vec(T **dst, const std::vector<T> &v)
+{
+    if (dmalloc(dst, v.size()))
+        return -1;
+    if (!v.empty())
+        HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+/* checks that need no device: one state count, in-degrees within a wave's
+ * chunk, observations after the parameters, complete tables */
+static int diff_validate(const bp_bn *bn, unsigned npd, Graph_t g)
+{
+    unsigned V = g->current_num_vertices;
+    unsigned S = V ? g->node_num_vars[0] : 1;
+    const unsigned R = S <= 4 ? (unsigned)BP_SLOTS_SMALL : 1u;
+    for (unsigned v = 0; v < V; ++v)
+        if (g->node_num_vars[v] != S) {
+            bp_set_error("bp_diff_create: node %u has %u states, node 0 has %u; the adjoint needs one state count", v,
+                         g->node_num_vars[v], S);
+            return -1;
+        }
+    for (unsigned v = 0; v < V; ++v) {
+        unsigned deg = bp_csr_end(g, g->dest_nodes_to_edges_node_list, v) - g->dest_nodes_to_edges_node_list[v];
+        if (deg > WAVE * R) {
+            bp_set_error("bp_diff_create: node %u has %u in-edges (the adjoint handles up to %u)", v, deg, WAVE * R);
+            return -1;
+        }
+    }
+    if (bn->def_table_begin[npd] > 0xfffffff0ull) {
+        bp_set_error("bp_diff_create: too many parameters");
+        return -1;
+    }
+    for (unsigned dd = 0; dd < bn->num_defs; ++dd) {
+        uint64_t g0 = bn->def_given_begin[dd], g1 = bn->def_given_begin[dd + 1];
+        uint64_t t0 = bn->def_table_begin[dd], t1 = bn->def_table_begin[dd + 1];
+        if (dd >= npd) {
+            if (g1 != g0) {
+                bp_set_error("bp_diff_create: definition %u after the parameters has GIVENs", dd);
+                return -1;
+            }
+            continue;
+        }
+        uint64_t expect = bn->num_states[bn->def_for[dd]];
+        for (uint64_t i = g0; i < g1; ++i)
+            expect *= bn->num_states[bn->def_given[i]];
+        if (t1 - t0 != expect) {
+            bp_set_error("bp_diff_create: definition %u has %llu TABLE entries, %llu expected", dd,
+                         (unsigned long long)(t1 - t0), (unsigned long long)expect);
+            return -1;
+        }
+    }
+    return 0;
+}
+
+static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
+{
+    bp_dev *d = x->d;
+    unsigned V = d->V, E = d->E, S = d->S;
+    x->num_params = (unsigned)bn->def_table_begin[npd];
+
+    /* replay the reader's edge emission: definitions in order, parents last
+     * GIVEN first, forward edge then reverse edge unless the parent is
+     * observed (bp_graph_add_cpt_edges) */
+    std::vector<JMap> jm(E);
+    std::vector<unsigned> def_of(x->num_params), def_tb(npd), def_slice(npd), def_pbegin(npd + 1, 0), def_prior(npd, NONE);
+    std::vector<ParentRec> prs;
+    unsigned e = 0;
+    for (unsigned dd = 0; dd < bn->num_defs; ++dd) {
+        uint64_t g0 = bn->def_given_begin[dd], g1 = bn->def_given_begin[dd + 1];
+        uint64_t t0 = bn->def_table_begin[dd], t1 = bn->def_table_begin[dd + 1];
+        unsigned child = bn->def_for[dd];
+        unsigned nch = bn->num_states[child];
+        if (dd >= npd)
+            continue;
+        def_tb[dd] = (unsigned)t0;
+        unsigned slice = (unsigned)((t1 - t0) / nch);
+        def_slice[dd] = slice;
+        for (uint64_t p = t0; p < t1; ++p)
+            def_of[p] = dd;
+        unsigned offset = 1;
+        for (uint64_t pi = g1; pi-- > g0;) {
+            unsigned parent = bn->def_given[pi];
+            unsigned npar = bn->num_states[parent];
+            ParentRec pr{offset, npar, NONE, NONE};
+            if (e >= E || g->edges_src_index[e] != parent || g->edges_dest_index[e] != child) {
+                bp_set_error("bp_diff_create: edge %u does not follow the reader's order", e);
+                return -1;
+            }
+            pr.slot_f = d->edge_slot[e];
+            jm[pr.slot_f] = JMap{(unsigned)t0, slice, offset, npar | (nch << 8)};
+            ++e;
+            if (g->observed_nodes[parent] != 1) {
+                if (e >= E || g->edges_src_index[e] != child || g->edges_dest_index[e] != parent) {
+                    bp_set_error("bp_diff_create: edge %u does not follow the reader's order", e);
+                    return -1;
+                }
+                pr.slot_r = d->edge_slot[e];
+                jm[pr.slot_r] = JMap{(unsigned)t0, slice, offset, npar | (nch << 8) | (1u << 16)};
+                ++e;
+            }
+            prs.push_back(pr);
+            offset *= npar;
+        }
+        def_pbegin[dd + 1] = (unsigned)prs.size();
+    }
+    if (e != E) {
+        bp_set_error("bp_diff_create: %u of %u edges come from the definitions", e, E);
+        return -1;
+    }
+    /* initial beliefs: the last GIVEN-less definition of a node sets them */
+    std::vector<unsigned> state_def(V, NONE), prior_tb(V, NONE);
+    for (unsigned dd = 0; dd < bn->num_defs; ++dd)
+        if (bn->def_given_begin[dd + 1] == bn->def_given_begin[dd])
+            state_def[bn->def_for[dd]] = dd;
+    for (unsigned v = 0; v < V; ++v)
+        if (state_def[v] != NONE && state_def[v] < npd) {
+            prior_tb[v] = def_tb[state_def[v]];
+            def_prior[state_def[v]] = v;
+        }
+    /* softmax rows */
+    x->row_first.clear();
+    x->row_stride.clear();
+    x->row_len.clear();
+    for (unsigned dd = 0; dd < npd; ++dd) {
+        unsigned nch = bn->num_states[bn->def_for[dd]];
+        for (unsigned cc = 0; cc < def_slice[dd]; ++cc) {
+            x->row_first.push_back((uint64_t)def_tb[dd] + cc);
+            x->row_stride.push_back(def_slice[dd]);
+            x->row_len.push_back(nch);
+        }
+    }
+    x->num_rows = (unsigned)x->row_first.size();
+    /* source-grouped slots */
+    std::vector<unsigned> out_off(V + 1, 0), out_slot(E), slot_src(E);
+    for (unsigned s = 0; s < E; ++s) {
+        slot_src[s] = g->edges_src_index[d->slot_edge[s]];
+        out_off[slot_src[s] + 1]++;
+    }
+    for (unsigned v = 0; v < V; ++v)
+        out_off[v + 1] += out_off[v];
+    {
+        std::vector<unsigned> fill(out_off.begin(), out_off.end() - 1);
+        for (unsigned s = 0; s < E; ++s)
+            out_slot[fill[slot_src[s]]++] = s;
+    }
+    size_t VS = (size_t)V * d->SP, ES = (size_t)E * d->SP;
+    if (upload_vec(&x->jmap, jm) || upload_vec(&x->prior_tb, prior_tb) || upload_vec(&x->out_off, out_off) ||
+        upload_vec(&x->out_slot, out_slot) || upload_vec(&x->slot_edge, d->slot_edge) ||
+        upload_vec(&x->def_of, def_of) || upload_vec(&x->def_tb, def_tb) || upload_vec(&x->def_slice, def_slice) ||
+        upload_vec(&x->def_pbegin, def_pbegin) || upload_vec(&x->def_prior, def_prior) ||
+        upload_vec(&x->parents, prs) || dmalloc(&x->P, x->num_params) || dmalloc(&x->gP, x->num_params) ||
+        dmalloc(&x->st0, VS) || dmalloc(&x->gin, VS) || dmalloc(&x->gst[0], VS) || dmalloc(&x->gst[1], VS) ||
+        dmalloc(&x->w[0], ES) || dmalloc(&x->w[1], ES) || dmalloc(&x->jbar, (size_t)E * S * S) ||
+        dmalloc(&x->dst0p, VS) || dmalloc(&x->dst0, VS))
+        return -1;
+    return 0;
+}
+
+extern "C" int bp_diff_create(const bp_bn *bn, unsigned int num_param_defs, int device, bp_diff **out)
+{
+    bp_clear_error();
+    if (!bn || !out) {
+        bp_set_error("bp_diff_create: NULL argument");
+        return -1;
+    }
+    *out = nullptr;
+    if (num_param_defs > bn->num_defs) {
+        bp_set_error("bp_diff_create: %u parameter definitions of %u", num_param_defs, bn->num_defs);
+        return -1;
+    }
+    int abort_prev = bp_set_abort_on_error(0);
+    Graph_t g = bp_bn_to_graph(bn);
+    if (g) {
+        set_up_src_nodes_to_edges(g);
+        set_up_dest_nodes_to_edges(g);
+    }
+    bp_set_abort_on_error(abort_prev);
+    if (!g || bp_has_error()) {
+        if (g)
+            graph_destroy(g);
+        return -1;
+    }
+    if (diff_validate(bn, num_param_defs, g)) {
+        graph_destroy(g);
+        return -1;
+    }
+    bp_diff *x = new bp_diff();
+    int rc = bp_upload_graph(g, device, &x->d);
+    if (rc == 0) {
+        x->V = x->d->V;
+        x->E = x->d->E;
+        x->S = x->d->S;
+        x->SP = x->d->SP;
+        rc = diff_build(x, bn, num_param_defs, g);
+    }
+    graph_destroy(g);
+    if (rc) {
+        bp_diff_free(x);
+        return -1;
+    }
+    *out = x;
+    return 0;
+}
+
+extern "C" int bp_diff_info(const bp_diff *x, uint64_t *info)
+{
+    if (!x || !info) {
+        bp_set_error("bp_diff_info: NULL argument");
+        return -1;
+    }
+    info[0] = x->V;
+    info[1] = x->E;
+    info[2] = x->S;
+    info[3] = x->SP;
+    info[4] = x->num_params;
+    info[5] = x->num_rows;
+    return 0;
+}
+
+extern "C" int bp_diff_rows(const bp_diff *x, uint64_t *row_first, uint32_t *row_stride, uint32_t *row_len)
+{
+    if (!x || !row_first || !row_stride || !row_len) {
+        bp_set_error("bp_diff_rows: NULL argument");
+        return -1;
+    }
+    std::copy(x->row_first.begin(), x->row_first.end(), row_first);
+    std::copy(x->row_stride.begin(), x->row_stride.end(), row_stride);
+    std::copy(x->row_len.begin(), x->row_len.end(), row_len);
+    return 0;
+}
+
+extern "C" int bp_diff_set_params(bp_diff *x, const float *params, void *stream)
+{
+    bp_clear_error();
+    if (!x || !params) {
+        bp_set_error("bp_diff_set_params: NULL argument");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(x->d->device));
+    if (x->num_params)
+        hipLaunchKernelGGL(k_copy, dim3(blocks_for(x->num_params)), dim3(256), 0, st, params, x->P,
+                           (size_t)x->num_params);
+    size_t nJ = (size_t)x->E * x->S * x->S;
+    if (nJ)
+        hipLaunchKernelGGL(k_tables, dim3(blocks_for(nJ)), dim3(256), 0, st, x->P, x->jmap, x->d->J, x->E, x->S);
+    size_t VS = (size_t)x->V * x->SP;
+    if (VS)
+        hipLaunchKernelGGL(k_initial, dim3(blocks_for(VS)), dim3(256), 0, st, x->P, x->prior_tb, x->d->states0,
+                           x->st0, x->V, x->S, x->SP);
+    HIP_TRY(hipGetLastError());
+    x->have_params = true;
+    x->have_forward = false;
+    return 0;
+}
+
+static int diff_storage(bp_diff *x, unsigned K, unsigned c)
+{
+    if (x->K == K && x->c == c && x->final_st)
+        return 0;
+    diff_release_storage(x);
+    size_t VS = (size_t)x->V * x->SP;
+    unsigned nck = K / c + 1;
+    if (dmalloc(&x->ck_st, nck * VS) || dmalloc(&x->ck_buf, nck * VS) || dmalloc(&x->seg_st, (size_t)c * VS) ||
+        dmalloc(&x->seg_buf, (size_t)c * VS) || dmalloc(&x->fw_st[0], VS) || dmalloc(&x->fw_st[1], VS) ||
+        dmalloc(&x->fw_buf[0], VS) || dmalloc(&x->fw_buf[1], VS) || dmalloc(&x->final_st, VS)) {
+        diff_release_storage(x);
+        return -1;
+    }
+    x->K = K;
+    x->c = c;
+    x->nck = nck;
+    return 0;
+}
+
+/* one FWD (or FINIT) launch: (st_in, buf_in) -> (st_out, buf_out) */
+static void fwd_step(bp_diff *x, int mode, const float *st_in, const float *buf_in, float *st_out, float *buf_out,
+                     hipStream_t st)
+{
+    LoopyArgs a = bp_make_args(x->d);
+    a.states = const_cast<float *>(st_in);
+    a.states_out = st_out;
+    a.buf_in = buf_in;
+    a.buf_out = buf_out;
+    a.msg_in = nullptr;
+    a.msg_out = nullptr;
+    a.partials = nullptr;
+    a.stop = nullptr;
+    bp_launch_loopy(x->d, mode, a, st);
+}
+
+static int capture_begin(bp_diff *x)
+{
+    HIP_TRY(hipStreamBeginCapture(x->d->stream, hipStreamCaptureModeThreadLocal));
+    return 0;
+}
+
+static int capture_end(bp_diff *x, hipGraphExec_t *exec)
+{
+    hipGraph_t graph = nullptr;
+    HIP_TRY(hipStreamEndCapture(x->d->stream, &graph));
+    hipError_t err = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (err != hipSuccess) {
+        bp_set_error("hipGraphInstantiate failed: %s", hipGetErrorString(err));
+        return -1;
+    }
+    return 0;
+}
+
+static void enqueue_forward(bp_diff *x, hipStream_t st)
+{
+    size_t VS = (size_t)x->V * x->SP;
+    unsigned K = x->K, c = x->c;
+    fwd_step(x, FINIT, x->st0, nullptr, nullptr, x->ck_buf, st);
+    const float *sp = x->st0, *bp = x->ck_buf;
+    for (unsigned t = 1; t <= K; ++t) {
+        float *so, *bo;
+        if (t % c == 0) {
+            so = x->ck_st + (size_t)(t / c) * VS;
+            bo = x->ck_buf + (size_t)(t / c) * VS;
+        } else {
+            so = x->fw_st[t & 1];
+            bo = x->fw_buf[t & 1];
+        }
+        fwd_step(x, FWD, sp, bp, so, bo, st);
+        sp = so;
+        bp = bo;
+    }
+    hipLaunchKernelGGL(k_copy, dim3(blocks_for(VS)), dim3(256), 0, st, sp, x->final_st, VS);
+}
+
+extern "C" int bp_diff_forward(bp_diff *x, unsigned int K, unsigned int c, float *beliefs, void *stream)
+{
+    bp_clear_error();
+    if (!x || !beliefs || c == 0) {
+        bp_set_error("bp_diff_forward: NULL argument or zero checkpoint interval");
+        return -1;
+    }
+    if (!x->have_params) {
+        bp_set_error("bp_diff_forward: bp_diff_set_params has not been called");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(x->d->device));
+    if (diff_storage(x, K, c))
+        return -1;
+    /* the ck_st[0] slot is the initial beliefs themselves */
+    size_t VS = (size_t)x->V * x->SP;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_copy, dim3(blocks_for(VS)), dim3(256), 0, st, x->st0, x->ck_st, VS);
+    auto key = std::make_pair(K, c);
+    auto it = x->fwd_graphs.find(key);
+    if (it == x->fwd_graphs.end()) {
+        hipGraphExec_t exec = nullptr;
+        if (capture_begin(x))
+            return -1;
+        enqueue_forward(x, x->d->stream);
+        if (capture_end(x, &exec))
+            return -1;
+        it = x->fwd_graphs.emplace(key, exec).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, st));
+    hipLaunchKernelGGL(k_copy, dim3(blocks_for(VS)), dim3(256), 0, st, x->final_st, beliefs, VS);
+    HIP_TRY(hipGetLastError());
+    x->have_forward = true;
+    return 0;
+}
+
+static void adj_step(bp_diff *x, int step0, const float *st_prev, const float *st_cur, const float *bsrc,
+                     const float *gst_in, float *gst_out, const float *w_in, float *w_out, int first, hipStream_t st)
+{
+    bp_dev *d = x->d;
+    AdjArgs a;
+    a.in_off = d->in_off;
+    a.slot_src = d->slot_src;
+    a.out_off = x->out_off;
+    a.out_slot = x->out_slot;
+    a.J = d->J;
+    a.chunk = d->chunk;
+    a.nchunks = d->nchunks;
+    a.st_prev = st_prev;
+    a.st_cur = st_cur;
+    a.bsrc = bsrc;
+    a.gst_in = gst_in;
+    a.gst_out = gst_out;
+    a.w_in = w_in;
+    a.w_out = w_out;
+    a.jbar = x->jbar;
+    a.jbar_first = first;
+    dim3 grid(d->G), block(BLOCK);
+#define BP_ADJ_CASE(SS)                                                                   \
+    case SS:                                                                              \
+        if (step0)                                                                        \
+            hipLaunchKernelGGL((k_adjoint<SS, 1>), grid, block, 0, st, a);                \
+        else                                                                              \
+            hipLaunchKernelGGL((k_adjoint<SS, 0>), grid, block, 0, st, a);                \
+        break;
+    switch (d->S) {
+        BP_ADJ_CASE(1)
+        BP_ADJ_CASE(2)
+        BP_ADJ_CASE(3)
+        BP_ADJ_CASE(4)
+        BP_ADJ_CASE(5)
+        BP_ADJ_CASE(6)
+        BP_ADJ_CASE(7)
+        BP_ADJ_CASE(8)
+        BP_ADJ_CASE(9)
+        BP_ADJ_CASE(10)
+    }
+#undef BP_ADJ_CASE
+}
+
+static void enqueue_backward(bp_diff *x, hipStream_t st)
+{
+    size_t VS = (size_t)x->V * x->SP;
+    unsigned K = x->K, c = x->c;
+    unsigned nseg = (K + c - 1) / c;
+    const float *gin = x->gin; /* dL/dst_K */
+    const float *w_in = nullptr;
+    int first = 1;
+    std::vector<const float *> stp(c + 1), bufp(c + 1);
+    for (unsigned k = nseg; k-- > 0;) {
+        unsigned t0 = k * c, t1 = std::min(t0 + c, K);
+        stp[0] = x->ck_st + (size_t)k * VS;
+        bufp[0] = x->ck_buf + (size_t)k * VS;
+        for (unsigned t = t0 + 1; t < t1; ++t) {
+            float *so = x->seg_st + (size_t)(t - t0) * VS, *bo = x->seg_buf + (size_t)(t - t0) * VS;
+            fwd_step(x, FWD, stp[t - 1 - t0], bufp[t - 1 - t0], so, bo, st);
+            stp[t - t0] = so;
+            bufp[t - t0] = bo;
+        }
+        stp[t1 - t0] = (t1 == K) ? x->final_st : x->ck_st + (size_t)(k + 1) * VS;
+        for (unsigned t = t1; t > t0; --t) {
+            float *gout = x->gst[(t - 1) & 1];
+            float *wout = x->w[t & 1];
+            adj_step(x, 0, stp[t - 1 - t0], stp[t - t0], bufp[t - 1 - t0], gin, gout, w_in, wout, first, st);
+            gin = gout;
+            w_in = wout;
+            first = 0;
+        }
+    }
+    /* step 0: init */
+    adj_step(x, 1, nullptr, x->st0, x->st0, gin, x->dst0p, w_in, x->w[0] == w_in ? x->w[1] : x->w[0], first, st);
+    const float *w0 = (x->w[0] == w_in) ? x->w[1] : x->w[0];
+    hipLaunchKernelGGL(k_initial_grad, dim3(blocks_for(x->V)), dim3(256), 0, st, x->dst0p, w0, x->out_off,
+                       x->out_slot, x->dst0, x->V, x->S, x->SP);
+    if (x->num_params)
+        hipLaunchKernelGGL(k_param_grad, dim3(blocks_for(x->num_params)), dim3(256), 0, st, x->jbar, x->dst0,
+                           x->def_of, x->def_tb, x->def_slice, x->def_pbegin, x->parents, x->def_prior, x->gP,
+                           x->num_params, x->S, x->SP);
+}
+
+extern "C" int bp_diff_backward_ex(bp_diff *x, const float *grad_beliefs, float *grad_params, float *grad_tables,
+                                   float *grad_initial, void *stream)
+{
+    bp_clear_error();
+    if (!x || !grad_beliefs) {
+        bp_set_error("bp_diff_backward: NULL argument");
+        return -1;
+    }
+    if (!x->have_forward) {
+        bp_set_error("bp_diff_backward: no forward since the parameters were set");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(x->d->device));
+    hipStream_t st = (hipStream_t)stream;
+    size_t VS = (size_t)x->V * x->SP;
+    hipLaunchKernelGGL(k_copy, dim3(blocks_for(VS)), dim3(256), 0, st, grad_beliefs, x->gin, VS);
+    auto key = std::make_pair(x->K, x->c);
+    auto it = x->bwd_graphs.find(key);
+    if (it == x->bwd_graphs.end()) {
+        hipGraphExec_t exec = nullptr;
+        if (capture_begin(x))
+            return -1;
+        enqueue_backward(x, x->d->stream);
+        if (capture_end(x, &exec))
+            return -1;
+        it = x->bwd_graphs.emplace(key, exec).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, st));
+    if (grad_params && x->num_params)
+        hipLaunchKernelGGL(k_copy, dim3(blocks_for(x->num_params)), dim3(256), 0, st, x->gP, grad_params,
+                           (size_t)x->num_params);
+    size_t SS = (size_t)x->S * x->S;
+    if (grad_tables && x->E)
+        hipLaunchKernelGGL(k_slot_to_edge, dim3(blocks_for((size_t)x->E * SS)), dim3(256), 0, st, x->jbar,
+                           x->slot_edge, grad_tables, x->E, (unsigned)SS);
+    if (grad_initial)
+        hipLaunchKernelGGL(k_copy, dim3(blocks_for(VS)), dim3(256), 0, st, x->dst0, grad_initial, VS);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int bp_diff_backward(bp_diff *x, const float *grad_beliefs, float *grad_params, void *stream)
+{
+    return bp_diff_backward_ex(x, grad_beliefs, grad_params, nullptr, nullptr, stream);
+}
+
+extern "C" int bp_diff_tables(bp_diff *x, float *out)
+{
+    bp_clear_error();
+    if (!x || !out) {
+        bp_set_error("bp_diff_tables: NULL argument");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(x->d->device));
+    HIP_TRY(hipDeviceSynchronize());
+    size_t SS = (size_t)x->S * x->S;
+    std::vector<float> tmp((size_t)x->E * SS);
+    if (!tmp.empty())
+        HIP_TRY(hipMemcpy(tmp.data(), x->d->J, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (unsigned s = 0; s < x->E; ++s)
+        std::memcpy(out + (size_t)x->d->slot_edge[s] * SS, tmp.data() + (size_t)s * SS, SS * sizeof(float));
+    return 0;
+}

@@ -69,8 +69,12 @@ constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
  * PREP  the combine buffer from given states and messages
  * EDGE  one edge-centric iteration (graph.c:1186-1227, defects fixed):
  *       messages from the sources' beliefs, beliefs multiplied by every
- *       in-message without NaN skip, divided by their sum when it is > 0 */
-enum Mode { ITER = 0, INIT = 1, PREP = 2, EDGE = 3 };
+ *       in-message without NaN skip, divided by their sum when it is > 0
+ * FWD   the beliefs and combine buffer of ITER without storing messages or
+ *       the delta, reading states and writing states_out (the unrolled
+ *       forward of the adjoint, which keeps every iterate it needs)
+ * FINIT the combine buffer of INIT without storing messages */
+enum Mode { ITER = 0, INIT = 1, PREP = 2, EDGE = 3, FWD = 4, FINIT = 5 };
 
 struct LoopyArgs {
     const unsigned *in_off;      /* [V+1] */
@@ -78,8 +82,8 @@ struct LoopyArgs {
     const unsigned char *slot_xy;/* [E] x | y << 4 (mixed networks only) */
     const float *J;              /* [E][S*S] */
     const unsigned char *nvars;  /* [V] (mixed networks only) */
-    float *states;               /* [V][SP] (EDGE: the beliefs before the iteration) */
-    float *states_out;           /* [V][SP] EDGE: the beliefs after it */
+    float *states;               /* [V][SP] (EDGE, FWD: the beliefs before the iteration) */
+    float *states_out;           /* [V][SP] EDGE, FWD: the beliefs after it */
     const float *buf_in;         /* [V][SP] gather source (ITER) */
     float *buf_out;              /* [V][SP] */
     const float *msg_in;         /* [E][SP] old messages (ITER) / messages (PREP) */
@@ -221,7 +225,7 @@ __device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, 
         x = xy & 15u;
         y = xy >> 4;
     }
-    const float *src = (MODE == INIT || MODE == EDGE) ? a.states : a.buf_in;
+    const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
     float b[S];
 #pragma unroll
     for (int j = 0; j < S; ++j)
@@ -237,7 +241,7 @@ __device__ __forceinline__ void slot_message(const LoopyArgs &a, unsigned s, flo
     constexpr int SP = Stride<S>::value;
     unsigned x;
     compute_message<S, MIXED, MODE>(a, s, m, x);
-    if (MODE == PREP)
+    if (MODE == PREP || MODE == FWD || MODE == FINIT)
         return;
     if (MODE == ITER || MODE == EDGE) {
         const float *old = a.msg_in + (size_t)s * SP;
@@ -334,6 +338,11 @@ __device__ __forceinline__ void write_node(const LoopyArgs &a, unsigned v, const
             a.states_out[(size_t)v * SP + k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
         return;
     }
+    if (MODE == FWD) {
+#pragma unroll
+        for (int k = 0; k < SP; ++k)
+            a.states_out[(size_t)v * SP + k] = (k < S && (!MIXED || (unsigned)k < n)) ? st[k] : 0.0f;
+    }
     if (MODE == ITER) {
         if (MIXED) {
 #pragma unroll
@@ -401,7 +410,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 load_row_nt<S * S>(a.J + (size_t)s * S * S, J[r]);
                 xy[r] = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
             }
-            if (MODE != INIT)
+            if (MODE == ITER || MODE == EDGE || MODE == PREP)
                 load_row_nt<S>(a.msg_in + (size_t)s * SP, old[r]);
         }
         unsigned v = nb + r * WAVE + lane;
@@ -409,7 +418,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
             q0[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v) : a.in_off[v]) - sb;
             q1[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v + 1) : a.in_off[v + 1]) - sb;
             /* INIT and EDGE gather beliefs: keep them cached there */
-            if (MODE == ITER || MODE == PREP)
+            if (MODE == ITER || MODE == PREP || MODE == FWD)
                 load_row_nt<S>(a.states + (size_t)v * SP, st[r]);
             else
                 load_row<S>(a.states + (size_t)v * SP, st[r]);
@@ -427,7 +436,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 for (int k = 0; k < S; ++k)
                     m[k] = old[r][k];
             } else {
-                const float *src = (MODE == INIT || MODE == EDGE) ? a.states : a.buf_in;
+                const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
                 float b[S];
                 load_row<S>(src + (size_t)u[r] * SP, b);
                 unsigned x = xy[r] & 15u, y = xy[r] >> 4;
@@ -443,11 +452,13 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                         }
                     }
                 }
-                float row[SP];
+                if (MODE == ITER || MODE == INIT || MODE == EDGE) {
+                    float row[SP];
 #pragma unroll
-                for (int i = 0; i < SP; ++i)
-                    row[i] = (i < S) ? m[i] : 0.0f;
-                store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
+                    for (int i = 0; i < SP; ++i)
+                        row[i] = (i < S) ? m[i] : 0.0f;
+                    store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
+                }
             }
 #pragma unroll
             for (int k = 0; k < S; ++k)
@@ -462,7 +473,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
         if (v < ne) {
             unsigned n = nn[r];
             float m[S], b[S];
-            if (MODE == ITER) {
+            if (MODE == ITER || MODE == FWD) {
                 float t[S];
 #pragma unroll
                 for (int k = 0; k < S; ++k)
@@ -544,10 +555,10 @@ __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned
 #pragma unroll
     for (int k = 0; k < S; ++k) {
         st[k] = a.states[(size_t)v * SP + k];
-        acc[k] = (MODE == ITER) ? 1.0f : st[k];
+        acc[k] = (MODE == ITER || MODE == FWD) ? 1.0f : st[k];
     }
     walk_wide<S, MIXED, MODE>(a, sb, se, lds, lane, dacc, true, acc, n);
-    if (MODE == ITER) {
+    if (MODE == ITER || MODE == FWD) {
 #pragma unroll
         for (int k = 0; k < S; ++k)
             if (!MIXED || (unsigned)k < n)
