@@ -202,6 +202,20 @@ _SIGNATURES = {
     "bp_diff_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bp_diff_backward_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bp_diff_tables": (C.c_int, [C.c_void_p, f32p]),
+    # bp_batch.h
+    "bp_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "bp_batch_free": (None, [C.c_void_p]),
+    "bp_batch_add": (C.c_int, [C.c_void_p, GraphP, C.POINTER(C.c_uint)]),
+    "bp_batch_upload": (C.c_int, [C.c_void_p, C.c_int]),
+    "bp_batch_set_config": (C.c_int, [C.c_void_p, C.c_uint, C.c_uint]),
+    "bp_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
+    "bp_batch_tune": (C.c_int, [C.c_void_p, C.c_uint, C.c_char_p]),
+    "bp_batch_init": (C.c_int, [C.c_void_p]),
+    "bp_batch_iterate": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_double)]),
+    "bp_batch_download": (C.c_int, [C.c_void_p, C.c_uint, GraphP]),
+    "bp_batch_sync": (C.c_int, [C.c_void_p]),
+    "bp_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "bp_batch_profile": (C.c_int, [C.c_void_p, C.c_uint, f32p]),
 }
 
 _lib = None
@@ -353,6 +367,76 @@ def bn_to_graph(bn, csr: bool = True, levels: bool = False) -> GraphP:
     if levels:
         lib.init_levels_to_nodes(g)
     return g
+
+
+class Batch:
+    """Many networks run together (bp_batch); frees itself."""
+
+    def __init__(self):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.bp_batch_create(C.byref(h)), "bp_batch_create")
+        self.h = h
+        self.n = 0
+
+    def add(self, g: GraphP) -> int:
+        i = C.c_uint()
+        check(self.lib.bp_batch_add(self.h, g, C.byref(i)), "bp_batch_add")
+        self.n += 1
+        return i.value
+
+    def upload(self, device: int = -1):
+        check(self.lib.bp_batch_upload(self.h, device), "bp_batch_upload")
+
+    def config(self) -> tuple[int, int]:
+        r, q = C.c_uint(), C.c_uint()
+        check(self.lib.bp_batch_get_config(self.h, C.byref(r), C.byref(q)), "bp_batch_get_config")
+        return r.value, q.value
+
+    def set_config(self, slots_per_lane: int, queue_block: int):
+        check(self.lib.bp_batch_set_config(self.h, slots_per_lane, queue_block), "bp_batch_set_config")
+
+    def tune(self, iterations: int = 4, cache: str | None = None):
+        check(self.lib.bp_batch_tune(self.h, iterations, cache.encode() if cache else None), "bp_batch_tune")
+
+    def init(self):
+        check(self.lib.bp_batch_init(self.h), "bp_batch_init")
+
+    def iterate(self, n: int, want_deltas: bool = False):
+        if want_deltas:
+            d = np.zeros(max(n, 1) * self.n, dtype=np.float64)
+            check(self.lib.bp_batch_iterate(self.h, n, d.ctypes.data_as(C.POINTER(C.c_double))), "bp_batch_iterate")
+            return d[: n * self.n].reshape(n, self.n)
+        check(self.lib.bp_batch_iterate(self.h, n, None), "bp_batch_iterate")
+        return None
+
+    def download(self, index: int, g: GraphP):
+        check(self.lib.bp_batch_download(self.h, index, g), "bp_batch_download")
+
+    def sync(self):
+        check(self.lib.bp_batch_sync(self.h), "bp_batch_sync")
+
+    def info(self) -> dict:
+        a = (C.c_uint64 * 8)()
+        check(self.lib.bp_batch_info(self.h, a), "bp_batch_info")
+        return {"instances": a[0], "V": a[1], "E": a[2], "S": a[3], "table_floats": a[4], "chunks": a[5],
+                "alg_bytes": a[6]}
+
+    def profile(self, n: int) -> float:
+        ms = C.c_float()
+        check(self.lib.bp_batch_profile(self.h, n, C.byref(ms)), "bp_batch_profile")
+        return ms.value
+
+    def free(self):
+        if self.h:
+            self.lib.bp_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Device:

@@ -1,0 +1,852 @@
+/*
+ * bp_batch.hip -- many networks per launch.
+ *
+ * Layout: the batch is the disjoint union of its instances.  Nodes and slots
+ * are numbered instance after instance; within an instance, slots follow
+ * the reference's destination-grouped edge order (src/graph/graph.c:285-313)
+ * as in bp_upload.  Tables live in one pool of distinct tables
+ * (row-major x by y floats); a slot points at its table, or at a stored
+ * transpose of it, so every arc's table is stored once for the forward edge,
+ * the reverse edge and every evidence set.  Chunks never span two instances,
+ * so each chunk's FP64 delta partial belongs to one instance and k_batch_finish
+ * sums them per instance in chunk order (deterministic, no atomics).
+ *
+ * The iteration kernel is k_loopy with packed tables and mixed state counts;
+ * its arithmetic per node and slot is the plain path's, so every instance
+ * gets the same bits as when it runs alone.
+ */
+#include <hip/hip_ext.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bp_dev.h"
+#include "../../include/bp_batch.h"
+
+using namespace bpk;
+
+namespace {
+
+__global__ void k_copy_batch(const float *__restrict__ src, float *__restrict__ dst, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = src[i];
+}
+
+__global__ void k_zero_counter(unsigned *q)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *q = 0u;
+}
+
+/* deltas[i] = sum of instance i's chunk partials, in chunk order */
+__global__ void k_batch_finish(const double *__restrict__ partials, const unsigned *__restrict__ inst_chunk,
+                               unsigned n_inst, double *__restrict__ deltas)
+{
+    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_inst)
+        return;
+    double t = 0.0;
+    for (unsigned c = inst_chunk[i]; c < inst_chunk[i + 1]; ++c)
+        t += partials[c];
+    deltas[i] = t;
+}
+
+inline unsigned nblocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+uint64_t hash_table(unsigned x, unsigned y, const float *T, bool transposed)
+{
+    /* FNV-1a over (rows, cols, the entries row by row) */
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](uint32_t v) {
+        for (int k = 0; k < 4; ++k) {
+            h ^= (v >> (8 * k)) & 0xffu;
+            h *= 1099511628211ull;
+        }
+    };
+    unsigned rows = transposed ? y : x, cols = transposed ? x : y;
+    mix(rows);
+    mix(cols);
+    for (unsigned i = 0; i < rows; ++i)
+        for (unsigned j = 0; j < cols; ++j) {
+            float v = transposed ? T[j * MAX_STATES + i] : T[i * MAX_STATES + j];
+            uint32_t u;
+            std::memcpy(&u, &v, 4);
+            mix(u);
+        }
+    return h;
+}
+
+struct Shape {
+    unsigned R = 1;
+    std::vector<uint2> chunks;
+    std::vector<unsigned> inst_chunk; /* [instances+1] */
+    uint2 *d_chunk = nullptr;
+    unsigned *d_inst_chunk = nullptr;
+    unsigned nchunks = 0;
+};
+
+struct UniqueTable {
+    unsigned off, x, y;
+};
+
+} // namespace
+
+struct bp_batch {
+    /* host staging */
+    unsigned S = 1;
+    std::vector<unsigned> node_off{0}, slot_off{0};
+    std::vector<unsigned char> nvars;
+    std::vector<float> st_stage;   /* [V][MAX_STATES] */
+    std::vector<unsigned> in_off;  /* [V] global */
+    std::vector<unsigned> slot_src, slot_tab, slot_edge;
+    std::vector<unsigned char> slot_xy;
+    std::vector<unsigned> edge_x;  /* per slot: x of the slot's edge (for downloads) */
+    std::vector<float> tabs;
+    std::vector<UniqueTable> uniq;
+    std::unordered_map<uint64_t, unsigned> lookup; /* hash -> index in uniq */
+    double alg_bytes = 0.0;
+    bool uploaded = false;
+
+    /* device */
+    int device = 0;
+    hipStream_t stream = nullptr;
+    unsigned V = 0, E = 0, SP = 1, n_inst = 0;
+    int cus = 256;
+    unsigned *d_in_off = nullptr, *d_slot_src = nullptr, *d_slot_tab = nullptr, *d_queue = nullptr;
+    unsigned char *d_slot_xy = nullptr, *d_nvars = nullptr;
+    float *d_tabs = nullptr, *d_states = nullptr, *d_states0 = nullptr, *d_buf[2] = {nullptr, nullptr};
+    float *d_msg[2] = {nullptr, nullptr};
+    double *d_partials = nullptr;
+    double *d_deltas = nullptr;
+    unsigned deltas_cap = 0;
+    Shape shapes[2];
+
+    /* configuration and run state */
+    unsigned R = 1, qblock = 0;
+    int latest = 0, bcur = 0;
+    unsigned swaps = 0;
+    bool ready = false;
+    std::map<std::string, hipGraphExec_t> graphs;
+};
+
+static void batch_clear_graphs(bp_batch *b)
+{
+    for (auto &kv : b->graphs)
+        (void)hipGraphExecDestroy(kv.second);
+    b->graphs.clear();
+}
+
+extern "C" int bp_batch_create(bp_batch **out)
+{
+    bp_clear_error();
+    if (!out) {
+        bp_set_error("bp_batch_create: NULL argument");
+        return -1;
+    }
+    *out = new bp_batch();
+    return 0;
+}
+
+extern "C" void bp_batch_free(bp_batch *b)
+{
+    if (!b)
+        return;
+    if (b->uploaded) {
+        (void)hipSetDevice(b->device);
+        if (b->stream)
+            (void)hipStreamSynchronize(b->stream);
+        batch_clear_graphs(b);
+        void *ps[] = {b->d_in_off, b->d_slot_src, b->d_slot_tab, b->d_queue, b->d_slot_xy, b->d_nvars, b->d_tabs,
+                      b->d_states, b->d_states0, b->d_buf[0], b->d_buf[1], b->d_msg[0], b->d_msg[1],
+                      b->d_partials, b->d_deltas, b->shapes[0].d_chunk, b->shapes[0].d_inst_chunk,
+                      b->shapes[1].d_chunk, b->shapes[1].d_inst_chunk};
+        for (void *p : ps)
+            if (p)
+                (void)hipFree(p);
+        if (b->stream)
+            (void)hipStreamDestroy(b->stream);
+    }
+    delete b;
+}
+
+/* index of a distinct table equal to T (x rows, y cols, MAX_STATES-strided),
+ * or to its transpose (*tr = 1); appends it when new */
+static unsigned intern_table(bp_batch *b, unsigned x, unsigned y, const float *T, bool *tr)
+{
+    auto same = [&](const UniqueTable &u, bool transposed) {
+        unsigned rows = transposed ? y : x, cols = transposed ? x : y;
+        if (u.x != rows || u.y != cols)
+            return false;
+        for (unsigned i = 0; i < rows; ++i)
+            for (unsigned j = 0; j < cols; ++j) {
+                float v = transposed ? T[j * MAX_STATES + i] : T[i * MAX_STATES + j];
+                if (std::memcmp(&v, &b->tabs[u.off + i * cols + j], 4) != 0)
+                    return false;
+            }
+        return true;
+    };
+    uint64_t h = hash_table(x, y, T, false);
+    auto it = b->lookup.find(h);
+    if (it != b->lookup.end() && same(b->uniq[it->second], false)) {
+        *tr = false;
+        return it->second;
+    }
+    uint64_t ht = hash_table(x, y, T, true);
+    auto jt = b->lookup.find(ht);
+    if (jt != b->lookup.end() && same(b->uniq[jt->second], true)) {
+        *tr = true;
+        return jt->second;
+    }
+    UniqueTable u{(unsigned)b->tabs.size(), x, y};
+    for (unsigned i = 0; i < x; ++i)
+        for (unsigned j = 0; j < y; ++j)
+            b->tabs.push_back(T[i * MAX_STATES + j]);
+    b->uniq.push_back(u);
+    unsigned idx = (unsigned)b->uniq.size() - 1;
+    if (it == b->lookup.end())
+        b->lookup.emplace(h, idx);
+    b->alg_bytes += 4.0 * x * y;
+    *tr = false;
+    return idx;
+}
+
+extern "C" int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index)
+{
+    bp_clear_error();
+    if (!b || !g) {
+        bp_set_error("bp_batch_add: NULL argument");
+        return -1;
+    }
+    if (b->uploaded) {
+        bp_set_error("bp_batch_add: the batch is already on the device");
+        return -1;
+    }
+    unsigned V = g->current_num_vertices, E = g->current_num_edges;
+    if (V != g->total_num_vertices) {
+        bp_set_error("bp_batch_add: graph has %u of %u nodes", V, g->total_num_vertices);
+        return -1;
+    }
+    for (unsigned v = 0; v < V; ++v) {
+        unsigned bgn = g->dest_nodes_to_edges_node_list[v], end = bp_csr_end(g, g->dest_nodes_to_edges_node_list, v);
+        if (bgn > end || end > E) {
+            bp_set_error("bp_batch_add: destination CSR is not set up (call set_up_dest_nodes_to_edges)");
+            return -1;
+        }
+        for (unsigned k = bgn; k < end; ++k) {
+            unsigned e = g->dest_nodes_to_edges_edge_list[k];
+            if (e >= E || g->edges_dest_index[e] != v) {
+                bp_set_error("bp_batch_add: destination CSR is not set up (call set_up_dest_nodes_to_edges)");
+                return -1;
+            }
+        }
+    }
+    if (V > 0 && bp_csr_end(g, g->dest_nodes_to_edges_node_list, V - 1) != E) {
+        bp_set_error("bp_batch_add: destination CSR does not cover all %u edges", E);
+        return -1;
+    }
+    uint64_t nv = (uint64_t)b->node_off.back() + V, ne = (uint64_t)b->slot_off.back() + E;
+    if (nv >= 0xffffffffull || ne >= 0xffffffffull) {
+        bp_set_error("bp_batch_add: the batch exceeds 32-bit node or edge indices");
+        return -1;
+    }
+    for (unsigned v = 0; v < V; ++v)
+        if (g->node_num_vars[v] == 0 || g->node_num_vars[v] > MAX_STATES) {
+            bp_set_error("bp_batch_add: node %u has %u states", v, g->node_num_vars[v]);
+            return -1;
+        }
+    for (unsigned e = 0; e < E; ++e) {
+        unsigned s = g->edges_src_index[e], t = g->edges_dest_index[e];
+        if (s >= V || t >= V || g->edges_x_dim[e] != g->node_num_vars[s] || g->edges_y_dim[e] != g->node_num_vars[t]) {
+            bp_set_error("bp_batch_add: edge %u is inconsistent with its nodes", e);
+            return -1;
+        }
+    }
+    unsigned n0 = b->node_off.back(), s0 = b->slot_off.back();
+    for (unsigned v = 0; v < V; ++v) {
+        unsigned n = g->node_num_vars[v];
+        b->S = std::max(b->S, n);
+        b->nvars.push_back((unsigned char)n);
+        for (unsigned k = 0; k < MAX_STATES; ++k)
+            b->st_stage.push_back(k < n ? g->node_states[(size_t)v * MAX_STATES + k] : 0.0f);
+        b->in_off.push_back(s0 + g->dest_nodes_to_edges_node_list[v]);
+        b->alg_bytes += 12.0 * n + 4.0;
+    }
+    for (unsigned k = 0; k < E; ++k) {
+        unsigned e = g->dest_nodes_to_edges_edge_list[k];
+        unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
+        bool tr = false;
+        unsigned u = intern_table(b, x, y, g->edges_joint_probabilities + (size_t)e * MAX_STATES * MAX_STATES, &tr);
+        unsigned off = b->uniq[u].off;
+        if (off >= 0x80000000u) {
+            bp_set_error("bp_batch_add: more than 2^31 distinct table entries");
+            return -1;
+        }
+        b->slot_src.push_back(n0 + g->edges_src_index[e]);
+        b->slot_tab.push_back(off | (tr ? 0x80000000u : 0u));
+        b->slot_xy.push_back((unsigned char)(x | (y << 4)));
+        b->slot_edge.push_back(e);
+        b->edge_x.push_back(x);
+        b->alg_bytes += 4.0 * (2.0 * x + y + 1.0);
+    }
+    b->node_off.push_back(n0 + V);
+    b->slot_off.push_back(s0 + E);
+    if (index)
+        *index = (unsigned)b->node_off.size() - 2;
+    return 0;
+}
+
+/* chunks per instance, never across instances (cap = 64 x slots per lane) */
+static void build_shape(bp_batch *b, Shape &sh, unsigned R)
+{
+    sh.R = R;
+    sh.chunks.clear();
+    sh.inst_chunk.assign(1, 0);
+    unsigned cap = WAVE * R;
+    unsigned ninst = (unsigned)b->node_off.size() - 1;
+    for (unsigned i = 0; i < ninst; ++i) {
+        unsigned v = b->node_off[i], vend = b->node_off[i + 1];
+        while (v < vend) {
+            unsigned s0 = b->in_off[v];
+            sh.chunks.push_back(make_uint2(v, s0));
+            unsigned w = v;
+            while (w < vend && w - v < cap && (w + 1 < b->V ? b->in_off[w + 1] : b->E) - s0 <= cap)
+                ++w;
+            if (w == v)
+                w = v + 1;
+            v = w;
+        }
+        sh.inst_chunk.push_back((unsigned)sh.chunks.size());
+    }
+    sh.nchunks = (unsigned)sh.chunks.size();
+    sh.chunks.push_back(make_uint2(b->V, b->E));
+}
+
+template <typename T> static int up(T **dst, const T *src, size_t n)
+{
+    if (dmalloc(dst, n))
+        return -1;
+    if (n)
+        HIP_TRY(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int bp_batch_upload(bp_batch *b, int device)
+{
+    bp_clear_error();
+    if (!b) {
+        bp_set_error("bp_batch_upload: NULL argument");
+        return -1;
+    }
+    if (b->uploaded) {
+        bp_set_error("bp_batch_upload: already uploaded");
+        return -1;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        bp_set_error("no HIP device available: the belief-propagation kernels need an MI355X (gfx950)");
+        return -1;
+    }
+    if (device < 0)
+        HIP_TRY(hipGetDevice(&device));
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        bp_set_error("device %d is %s; libbp_amd.so is built for gfx950 only", device, prop.gcnArchName);
+        return -1;
+    }
+    b->device = device;
+    b->cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    b->V = b->node_off.back();
+    b->E = b->slot_off.back();
+    b->n_inst = (unsigned)b->node_off.size() - 1;
+    b->SP = stride_of(b->S);
+    build_shape(b, b->shapes[0], 1);
+    if (b->S <= 6)
+        build_shape(b, b->shapes[1], 2);
+    b->R = (b->S <= 4) ? 2 : 1;
+    b->qblock = 0;
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->uploaded = true; /* from here on bp_batch_free releases device memory */
+
+    size_t VS = (size_t)b->V * b->SP, ES = (size_t)b->E * b->SP;
+    std::vector<float> st(VS, 0.0f);
+    for (size_t v = 0; v < b->V; ++v)
+        for (unsigned k = 0; k < b->nvars[v]; ++k)
+            st[v * b->SP + k] = b->st_stage[v * MAX_STATES + k];
+    std::vector<unsigned> in_off(b->in_off);
+    in_off.push_back(b->E);
+    unsigned maxchunks = std::max(b->shapes[0].nchunks, b->shapes[1].nchunks);
+    if (up(&b->d_in_off, in_off.data(), in_off.size()) || up(&b->d_slot_src, b->slot_src.data(), b->E) ||
+        up(&b->d_slot_tab, b->slot_tab.data(), b->E) || up(&b->d_slot_xy, b->slot_xy.data(), b->E) ||
+        up(&b->d_nvars, b->nvars.data(), b->V) || up(&b->d_tabs, b->tabs.data(), b->tabs.size()) ||
+        up(&b->d_states0, st.data(), VS) || dmalloc(&b->d_states, VS) || dmalloc(&b->d_buf[0], VS) ||
+        dmalloc(&b->d_buf[1], VS) || dmalloc(&b->d_msg[0], ES) || dmalloc(&b->d_msg[1], ES) ||
+        dmalloc(&b->d_partials, std::max(maxchunks, 1u)) || dmalloc(&b->d_queue, 1))
+        return -1;
+    for (int k = 0; k < 2; ++k) {
+        Shape &sh = b->shapes[k];
+        if (sh.chunks.empty())
+            continue;
+        if (up(&sh.d_chunk, sh.chunks.data(), sh.chunks.size()) ||
+            up(&sh.d_inst_chunk, sh.inst_chunk.data(), sh.inst_chunk.size()))
+            return -1;
+    }
+    HIP_TRY(hipMemcpy(b->d_states, b->d_states0, VS * sizeof(float), hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemset(b->d_buf[0], 0, VS * sizeof(float)));
+    HIP_TRY(hipMemset(b->d_buf[1], 0, VS * sizeof(float)));
+    HIP_TRY(hipMemset(b->d_msg[0], 0, ES * sizeof(float)));
+    HIP_TRY(hipMemset(b->d_msg[1], 0, ES * sizeof(float)));
+    /* host staging no longer needed (downloads keep slot_edge, edge_x, offsets, nvars) */
+    std::vector<float>().swap(b->st_stage);
+    std::vector<unsigned>().swap(b->slot_src);
+    std::vector<unsigned>().swap(b->slot_tab);
+    std::vector<unsigned char>().swap(b->slot_xy);
+    std::vector<float>().swap(b->tabs);
+    b->lookup.clear();
+    return 0;
+}
+
+static bool valid_config(const bp_batch *b, unsigned R, unsigned qblock)
+{
+    (void)qblock;
+    /* two slots per lane would spill registers beyond six states */
+    return (R == 1) || (R == 2 && b->S <= 6);
+}
+
+extern "C" int bp_batch_set_config(bp_batch *b, unsigned int slots_per_lane, unsigned int queue_block)
+{
+    bp_clear_error();
+    if (!b || !b->uploaded) {
+        bp_set_error("bp_batch_set_config: batch not uploaded");
+        return -1;
+    }
+    if (!valid_config(b, slots_per_lane, queue_block)) {
+        bp_set_error("bp_batch_set_config: %u slots per lane is not available for %u states", slots_per_lane, b->S);
+        return -1;
+    }
+    b->R = slots_per_lane;
+    b->qblock = queue_block;
+    return 0;
+}
+
+extern "C" int bp_batch_get_config(const bp_batch *b, unsigned int *slots_per_lane, unsigned int *queue_block)
+{
+    if (!b || !slots_per_lane || !queue_block) {
+        bp_set_error("bp_batch_get_config: NULL argument");
+        return -1;
+    }
+    *slots_per_lane = b->R;
+    *queue_block = b->qblock;
+    return 0;
+}
+
+static const Shape &cur_shape(const bp_batch *b) { return b->shapes[b->R == 2 ? 1 : 0]; }
+
+static LoopyArgs batch_args(bp_batch *b)
+{
+    const Shape &sh = cur_shape(b);
+    LoopyArgs a;
+    a.in_off = b->d_in_off;
+    a.slot_src = b->d_slot_src;
+    a.slot_xy = b->d_slot_xy;
+    a.J = nullptr;
+    a.nvars = b->d_nvars;
+    a.states = b->d_states;
+    a.states_out = b->d_states;
+    a.buf_in = b->d_buf[b->bcur];
+    a.buf_out = b->d_buf[b->bcur ^ 1];
+    a.msg_in = b->d_msg[b->latest];
+    a.msg_out = b->d_msg[b->latest ^ 1];
+    a.chunk = sh.d_chunk;
+    a.nchunks = sh.nchunks;
+    a.partials = nullptr;
+    a.stop = nullptr;
+    a.slot_tab = b->d_slot_tab;
+    a.tabs = b->d_tabs;
+    a.chunk_partials = b->d_partials;
+    a.queue = b->qblock ? b->d_queue : nullptr;
+    a.qblock = b->qblock ? b->qblock : 1;
+    return a;
+}
+
+static unsigned batch_grid(const bp_batch *b)
+{
+    const Shape &sh = cur_shape(b);
+    unsigned want = (sh.nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    unsigned G = std::max(8u, (want + 7u) & ~7u);
+    if (b->qblock) /* persistent: eight workgroups (32 waves) per CU */
+        G = std::min(G, std::max(8u, (unsigned)b->cus * 8u));
+    return G;
+}
+
+template <int S, int R>
+static void launch_batch_sr(int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
+{
+    dim3 grid(G), block(BLOCK);
+    if (mode == ITER) {
+        if (ev)
+            hipExtLaunchKernelGGL((k_loopy<S, true, ITER, R, true>), grid, block, 0, st, ev[0], ev[1], 0, a);
+        else
+            hipLaunchKernelGGL((k_loopy<S, true, ITER, R, true>), grid, block, 0, st, a);
+    } else {
+        hipLaunchKernelGGL((k_loopy<S, true, INIT, R, true>), grid, block, 0, st, a);
+    }
+}
+
+template <int S>
+static void launch_batch_s(unsigned R, int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
+{
+    if constexpr (S <= 6) {
+        if (R == 2) {
+            launch_batch_sr<S, 2>(mode, a, G, st, ev);
+            return;
+        }
+    }
+    launch_batch_sr<S, 1>(mode, a, G, st, ev);
+}
+
+static void launch_batch(bp_batch *b, int mode, const LoopyArgs &a, hipStream_t st, const hipEvent_t *ev = nullptr)
+{
+    if (b->V == 0)
+        return;
+    if (b->qblock)
+        hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, b->d_queue);
+    unsigned G = batch_grid(b);
+    switch (b->S) {
+    case 1: launch_batch_s<1>(b->R, mode, a, G, st, ev); break;
+    case 2: launch_batch_s<2>(b->R, mode, a, G, st, ev); break;
+    case 3: launch_batch_s<3>(b->R, mode, a, G, st, ev); break;
+    case 4: launch_batch_s<4>(b->R, mode, a, G, st, ev); break;
+    case 5: launch_batch_s<5>(b->R, mode, a, G, st, ev); break;
+    case 6: launch_batch_s<6>(b->R, mode, a, G, st, ev); break;
+    case 7: launch_batch_s<7>(b->R, mode, a, G, st, ev); break;
+    case 8: launch_batch_s<8>(b->R, mode, a, G, st, ev); break;
+    case 9: launch_batch_s<9>(b->R, mode, a, G, st, ev); break;
+    default: launch_batch_s<10>(b->R, mode, a, G, st, ev); break;
+    }
+}
+
+extern "C" int bp_batch_init(bp_batch *b)
+{
+    bp_clear_error();
+    if (!b || !b->uploaded) {
+        bp_set_error("bp_batch_init: batch not uploaded");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(b->device));
+    size_t VS = (size_t)b->V * b->SP;
+    if (VS)
+        hipLaunchKernelGGL(k_copy_batch, dim3(nblocks(VS)), dim3(256), 0, b->stream, b->d_states0, b->d_states, VS);
+    b->latest = 0;
+    b->bcur = 0;
+    b->swaps = 0;
+    LoopyArgs a = batch_args(b);
+    a.msg_out = b->d_msg[b->latest];
+    a.buf_out = b->d_buf[b->bcur];
+    launch_batch(b, INIT, a, b->stream);
+    HIP_TRY(hipGetLastError());
+    b->ready = true;
+    return 0;
+}
+
+static int ensure_deltas(bp_batch *b, unsigned n)
+{
+    size_t need = (size_t)n * b->n_inst;
+    if (need <= b->deltas_cap)
+        return 0;
+    if (b->d_deltas)
+        HIP_TRY(hipFree(b->d_deltas));
+    b->d_deltas = nullptr;
+    batch_clear_graphs(b); /* captured graphs hold the old address */
+    if (dmalloc(&b->d_deltas, need))
+        return -1;
+    b->deltas_cap = (unsigned)need;
+    return 0;
+}
+
+extern "C" int bp_batch_iterate(bp_batch *b, unsigned int n, double *deltas)
+{
+    bp_clear_error();
+    if (!b || !b->uploaded || !b->ready) {
+        bp_set_error("bp_batch_iterate: call bp_batch_init first");
+        return -1;
+    }
+    if (n == 0)
+        return 0;
+    HIP_TRY(hipSetDevice(b->device));
+    if (ensure_deltas(b, n))
+        return -1;
+    char key[96];
+    snprintf(key, sizeof(key), "%u/%u/%u/%d/%d", n, b->R, b->qblock, b->latest, b->bcur);
+    auto it = b->graphs.find(key);
+    if (it == b->graphs.end()) {
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int lat = b->latest, bc = b->bcur;
+        const Shape &sh = cur_shape(b);
+        for (unsigned i = 0; i < n; ++i) {
+            LoopyArgs a = batch_args(b);
+            a.buf_in = b->d_buf[bc];
+            a.buf_out = b->d_buf[bc ^ 1];
+            a.msg_in = b->d_msg[lat];
+            a.msg_out = b->d_msg[lat ^ 1];
+            launch_batch(b, ITER, a, b->stream);
+            hipLaunchKernelGGL(k_batch_finish, dim3(nblocks(b->n_inst)), dim3(256), 0, b->stream, b->d_partials,
+                               sh.d_inst_chunk, b->n_inst, b->d_deltas + (size_t)i * b->n_inst);
+            lat ^= 1;
+            bc ^= 1;
+        }
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamEndCapture(b->stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        hipError_t err = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (err != hipSuccess) {
+            bp_set_error("hipGraphInstantiate failed: %s", hipGetErrorString(err));
+            return -1;
+        }
+        it = b->graphs.emplace(key, exec).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, b->stream));
+    if (n & 1u) {
+        b->latest ^= 1;
+        b->bcur ^= 1;
+    }
+    b->swaps += n;
+    if (deltas) {
+        HIP_TRY(hipMemcpyAsync(deltas, b->d_deltas, (size_t)n * b->n_inst * sizeof(double), hipMemcpyDeviceToHost,
+                               b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+    }
+    return 0;
+}
+
+extern "C" int bp_batch_sync(bp_batch *b)
+{
+    bp_clear_error();
+    if (!b || !b->uploaded) {
+        bp_set_error("bp_batch_sync: batch not uploaded");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+extern "C" int bp_batch_download(bp_batch *b, unsigned int index, Graph_t g)
+{
+    bp_clear_error();
+    if (!b || !g || !b->uploaded) {
+        bp_set_error("bp_batch_download: NULL argument or batch not uploaded");
+        return -1;
+    }
+    if (index >= b->n_inst) {
+        bp_set_error("bp_batch_download: instance %u of %u", index, b->n_inst);
+        return -1;
+    }
+    unsigned n0 = b->node_off[index], V = b->node_off[index + 1] - n0;
+    unsigned s0 = b->slot_off[index], E = b->slot_off[index + 1] - s0;
+    if (g->current_num_vertices != V || g->current_num_edges != E) {
+        bp_set_error("bp_batch_download: graph does not match instance %u", index);
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(b->device));
+    unsigned SP = b->SP;
+    std::vector<float> st((size_t)V * SP), m0((size_t)E * SP), m1((size_t)E * SP);
+    HIP_TRY(hipMemcpyAsync(st.data(), b->d_states + (size_t)n0 * SP, st.size() * 4, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipMemcpyAsync(m0.data(), b->d_msg[b->latest] + (size_t)s0 * SP, m0.size() * 4, hipMemcpyDeviceToHost,
+                           b->stream));
+    HIP_TRY(hipMemcpyAsync(m1.data(), b->d_msg[b->latest ^ 1] + (size_t)s0 * SP, m1.size() * 4,
+                           hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (unsigned v = 0; v < V; ++v)
+        for (unsigned k = 0; k < b->nvars[n0 + v]; ++k)
+            g->node_states[(size_t)v * MAX_STATES + k] = st[(size_t)v * SP + k];
+    if (b->swaps & 1u) {
+        float **t = g->previous_edge_messages;
+        g->previous_edge_messages = g->current_edge_messages;
+        g->current_edge_messages = t;
+    }
+    float *prev = *g->previous_edge_messages, *cur = *g->current_edge_messages;
+    for (unsigned k = 0; k < E; ++k) {
+        unsigned e = b->slot_edge[s0 + k], x = b->edge_x[s0 + k];
+        for (unsigned i = 0; i < x; ++i) {
+            prev[(size_t)e * MAX_STATES + i] = m0[(size_t)k * SP + i];
+            cur[(size_t)e * MAX_STATES + i] = m1[(size_t)k * SP + i];
+        }
+    }
+    return 0;
+}
+
+extern "C" int bp_batch_info(const bp_batch *b, uint64_t *info)
+{
+    if (!b || !info) {
+        bp_set_error("bp_batch_info: NULL argument");
+        return -1;
+    }
+    info[0] = b->node_off.size() - 1;
+    info[1] = b->node_off.back();
+    info[2] = b->slot_off.back();
+    info[3] = b->S;
+    uint64_t tf = 0;
+    for (const auto &u : b->uniq)
+        tf += (uint64_t)u.x * u.y;
+    info[4] = tf;
+    info[5] = b->uploaded ? cur_shape(b).nchunks : 0;
+    info[6] = (uint64_t)b->alg_bytes;
+    return 0;
+}
+
+extern "C" int bp_batch_profile(bp_batch *b, unsigned int n, float *ms)
+{
+    bp_clear_error();
+    if (!b || !ms || !b->uploaded || !b->ready || n == 0) {
+        bp_set_error("bp_batch_profile: bad arguments or batch not initialised");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(b->device));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float total = 0.0f;
+    for (unsigned i = 0; i < n; ++i) {
+        LoopyArgs a = batch_args(b);
+        hipEvent_t ev[2] = {e0, e1};
+        launch_batch(b, ITER, a, b->stream, ev);
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+        total += t;
+        b->latest ^= 1;
+        b->bcur ^= 1;
+        b->swaps++;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = total / n;
+    return 0;
+}
+
+/* ---- autotuner ------------------------------------------------------------- */
+
+static std::string batch_signature(const bp_batch *b)
+{
+    hipDeviceProp_t prop;
+    std::string dev = "gpu";
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess)
+        dev = prop.gcnArchName;
+    for (char &c : dev)
+        if (c == '"' || c == ':')
+            c = '_';
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%s/cu%d/inst%u/V%u/E%u/S%u", dev.c_str(), b->cus, b->n_inst, b->V, b->E, b->S);
+    return buf;
+}
+
+static bool cache_lookup(const char *path, const std::string &sig, unsigned *R, unsigned *q)
+{
+    if (!path)
+        return false;
+    FILE *f = fopen(path, "r");
+    if (!f)
+        return false;
+    std::string text;
+    char chunk[4096];
+    size_t n;
+    while ((n = fread(chunk, 1, sizeof(chunk), f)) > 0)
+        text.append(chunk, n);
+    fclose(f);
+    std::string key = "\"" + sig + "\"";
+    size_t pos = text.find(key);
+    if (pos == std::string::npos)
+        return false;
+    size_t r = text.find("\"slots_per_lane\":", pos), qq = text.find("\"queue_block\":", pos);
+    if (r == std::string::npos || qq == std::string::npos)
+        return false;
+    *R = (unsigned)strtoul(text.c_str() + r + 17, nullptr, 10);
+    *q = (unsigned)strtoul(text.c_str() + qq + 14, nullptr, 10);
+    return true;
+}
+
+static void cache_store(const char *path, const std::string &sig, unsigned R, unsigned q, double ms)
+{
+    if (!path)
+        return;
+    std::string text;
+    if (FILE *f = fopen(path, "r")) {
+        char chunk[4096];
+        size_t n;
+        while ((n = fread(chunk, 1, sizeof(chunk), f)) > 0)
+            text.append(chunk, n);
+        fclose(f);
+    }
+    size_t end = text.rfind('}');
+    char entry[512];
+    snprintf(entry, sizeof(entry), "  \"%s\": {\"slots_per_lane\": %u, \"queue_block\": %u, \"ms_per_iteration\": %.6f}",
+             sig.c_str(), R, q, ms);
+    std::string out;
+    if (end == std::string::npos || text.find('"') == std::string::npos)
+        out = std::string("{\n") + entry + "\n}\n";
+    else
+        out = text.substr(0, end) + ",\n" + entry + "\n}\n";
+    /* strip a blank line left between the previous entry and the comma */
+    size_t bad;
+    while ((bad = out.find("\n,\n")) != std::string::npos)
+        out.replace(bad, 3, ",\n");
+    if (FILE *f = fopen(path, "w")) {
+        fputs(out.c_str(), f);
+        fclose(f);
+    }
+}
+
+extern "C" int bp_batch_tune(bp_batch *b, unsigned int iterations, const char *cache_path)
+{
+    bp_clear_error();
+    if (!b || !b->uploaded) {
+        bp_set_error("bp_batch_tune: batch not uploaded");
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(b->device));
+    std::string sig = batch_signature(b);
+    unsigned R = 0, q = 0;
+    if (cache_lookup(cache_path, sig, &R, &q) && valid_config(b, R, q)) {
+        b->R = R;
+        b->qblock = q;
+        return 0;
+    }
+    const unsigned Rs[2] = {1, 2};
+    const unsigned Qs[4] = {0, 1, 4, 16};
+    double best = 1e300;
+    unsigned bestR = b->R, bestQ = b->qblock;
+    unsigned n = iterations ? iterations : 4;
+    for (unsigned R2 : Rs)
+        for (unsigned q2 : Qs) {
+            if (!valid_config(b, R2, q2))
+                continue;
+            b->R = R2;
+            b->qblock = q2;
+            if (bp_batch_init(b))
+                return -1;
+            float ms = 0.0f;
+            if (bp_batch_profile(b, 1, &ms) || bp_batch_profile(b, n, &ms))
+                return -1;
+            if (ms < best) {
+                best = ms;
+                bestR = R2;
+                bestQ = q2;
+            }
+        }
+    b->R = bestR;
+    b->qblock = bestQ;
+    b->ready = false; /* the timing runs changed the state: init again */
+    cache_store(cache_path, sig, bestR, bestQ, best);
+    return 0;
+}

@@ -443,6 +443,11 @@ LoopyArgs bp_make_args(bp_dev *d)
     a.nchunks = d->nchunks;
     a.partials = d->partials;
     a.stop = nullptr;
+    a.slot_tab = nullptr;
+    a.tabs = nullptr;
+    a.chunk_partials = nullptr;
+    a.queue = nullptr;
+    a.qblock = 1;
     return a;
 }
 

@@ -92,6 +92,14 @@ struct LoopyArgs {
     unsigned nchunks;
     double *partials;            /* [G] this launch's per-block delta partials (ITER) */
     const unsigned *stop;        /* non-null: skip the launch once *stop != 0 */
+    /* packed tables (batched networks): slot s's table starts at
+     * tabs + (slot_tab[s] & 0x7fffffff), row-major with the slot's x rows and
+     * y columns, or stored transposed (y rows, x columns) when bit 31 is set */
+    const unsigned *slot_tab;
+    const float *tabs;
+    double *chunk_partials;      /* non-null: one delta partial per chunk instead of per block */
+    unsigned *queue;             /* non-null: chunks are taken from this counter in blocks of qblock */
+    unsigned qblock;
 };
 
 template <int S> struct Stride {
@@ -206,8 +214,65 @@ __device__ __forceinline__ void send(const float *__restrict__ Jrow, const float
         m[i] = m[i] / sum;
 }
 
+/* The table of slot s as J[i*S + j] (zero beyond x rows and y columns). */
+template <int S, bool PACKED>
+__device__ __forceinline__ void load_table(const LoopyArgs &a, unsigned s, unsigned xy, float *J)
+{
+    if constexpr (!PACKED) {
+        load_row_nt<S * S>(a.J + (size_t)s * S * S, J);
+    } else {
+        unsigned t = a.slot_tab[s], x = xy & 15u, y = xy >> 4;
+        const float *T = a.tabs + (t & 0x7fffffffu);
+        bool tr = (t >> 31) != 0;
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+                J[i * S + j] = ((unsigned)i < x && (unsigned)j < y) ? T[tr ? j * x + i : i * y + j] : 0.0f;
+    }
+}
+
+/* m = N(J . b) with the table read row by row from memory instead of held
+ * in registers (large state counts); same operations in the same order as
+ * send(). */
+template <int S, bool MIXED, bool PACKED>
+__device__ __forceinline__ void send_from_memory(const LoopyArgs &a, unsigned s, unsigned xy, const float *b, float *m)
+{
+    unsigned x = MIXED ? (xy & 15u) : (unsigned)S, y = MIXED ? (xy >> 4) : (unsigned)S;
+    const float *T;
+    bool tr = false;
+    if constexpr (PACKED) {
+        unsigned t = a.slot_tab[s];
+        T = a.tabs + (t & 0x7fffffffu);
+        tr = (t >> 31) != 0;
+    } else {
+        T = a.J + (size_t)s * S * S;
+    }
+    /* row i starts at T + i*y (or T + i stored transposed), entries step 1 (or x) apart */
+    const unsigned rstep = PACKED ? (tr ? 1u : y) : (unsigned)S, cstep = (PACKED && tr) ? x : 1u;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        float acc = 0.0f;
+        if (!MIXED || (unsigned)i < x) {
+            const float *row = T + (size_t)i * rstep;
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+                if (!MIXED || (unsigned)j < y)
+                    acc = acc + row[(size_t)j * cstep] * b[j];
+            sum = sum + acc;
+        }
+        m[i] = acc;
+    }
+    if (sum <= 0.0f)
+        sum = 1.0f;
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+        m[i] = m[i] / sum;
+}
+
 /* The message of slot s into m[0..S) (zero beyond x), without side effects. */
-template <int S, bool MIXED, int MODE>
+template <int S, bool MIXED, int MODE, bool PACKED>
 __device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, float *m, unsigned &x)
 {
     constexpr int SP = Stride<S>::value;
@@ -230,17 +295,17 @@ __device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, 
 #pragma unroll
     for (int j = 0; j < S; ++j)
         b[j] = src[(size_t)u * SP + j];
-    send<S, MIXED>(a.J + (size_t)s * S * S, b, x, y, m);
+    send_from_memory<S, MIXED, PACKED>(a, s, x | (y << 4), b, m);
 }
 
 /* Slot phase: the message of slot s, written to msg_out (ITER, INIT, EDGE)
  * with its |new - old| added to dacc (ITER, EDGE). */
-template <int S, bool MIXED, int MODE>
+template <int S, bool MIXED, int MODE, bool PACKED>
 __device__ __forceinline__ void slot_message(const LoopyArgs &a, unsigned s, float *m, double &dacc)
 {
     constexpr int SP = Stride<S>::value;
     unsigned x;
-    compute_message<S, MIXED, MODE>(a, s, m, x);
+    compute_message<S, MIXED, MODE, PACKED>(a, s, m, x);
     if (MODE == PREP || MODE == FWD || MODE == FINIT)
         return;
     if (MODE == ITER || MODE == EDGE) {
@@ -393,22 +458,26 @@ __device__ __forceinline__ void load_row(const float *__restrict__ p, float *out
  * slots sb+l+64r and nodes nb+l+64r (r < R).  Every load that does not
  * depend on another one is issued before any is consumed, so a wave keeps
  * R slots' and R nodes' worth of reads in flight across one round trip. */
-template <int S, bool MIXED, int MODE, int R>
+template <int S, bool MIXED, int MODE, int R, bool PACKED>
 __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se,
                                              float *lds, unsigned lane, double &dacc)
 {
     constexpr int SP = Stride<S>::value;
     constexpr int W = WAVE * R;
+    /* small tables are loaded up front with the other streams; large ones
+     * are read row by row when the message is formed (register budget) */
+    constexpr bool HOLD = S * S * R <= 32;
     unsigned u[R], xy[R], q0[R], q1[R], nn[R];
-    float J[R][S * S], old[R][S], st[R][S];
+    float J[HOLD ? R : 1][HOLD ? S * S : 1], old[R][S], st[R][S];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         unsigned s = sb + r * WAVE + lane;
         if (s < se) {
             if (MODE != PREP) {
                 u[r] = BP_NT_LOADS ? __builtin_nontemporal_load(a.slot_src + s) : a.slot_src[s];
-                load_row_nt<S * S>(a.J + (size_t)s * S * S, J[r]);
                 xy[r] = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
+                if constexpr (HOLD)
+                    load_table<S, PACKED>(a, s, xy[r], J[r]);
             }
             if (MODE == ITER || MODE == EDGE || MODE == PREP)
                 load_row_nt<S>(a.msg_in + (size_t)s * SP, old[r]);
@@ -440,7 +509,10 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 float b[S];
                 load_row<S>(src + (size_t)u[r] * SP, b);
                 unsigned x = xy[r] & 15u, y = xy[r] >> 4;
-                send<S, MIXED>(J[r], b, x, y, m);
+                if constexpr (HOLD)
+                    send<S, MIXED>(J[r], b, x, y, m);
+                else
+                    send_from_memory<S, MIXED, PACKED>(a, s, xy[r], b, m);
                 if (MODE == ITER || MODE == EDGE) {
 #pragma unroll
                     for (int i = 0; i < S; ++i) {
@@ -513,7 +585,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
  * for the belief update, then the combine buffer from the updated state --
  * recomputing the messages on the second walk instead of reading back what
  * other lanes wrote. */
-template <int S, bool MIXED, int MODE>
+template <int S, bool MIXED, int MODE, bool PACKED>
 __device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *lds, unsigned lane, double &dacc,
                           bool first, float *acc, unsigned n)
 {
@@ -522,10 +594,10 @@ __device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *l
         unsigned s = base + lane;
         if (s < se) {
             if (first) {
-                slot_message<S, MIXED, MODE>(a, s, m, dacc);
+                slot_message<S, MIXED, MODE, PACKED>(a, s, m, dacc);
             } else {
                 unsigned x;
-                compute_message<S, MIXED, MODE>(a, s, m, x);
+                compute_message<S, MIXED, MODE, PACKED>(a, s, m, x);
             }
 #pragma unroll
             for (int k = 0; k < S; ++k)
@@ -545,7 +617,7 @@ __device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *l
     }
 }
 
-template <int S, bool MIXED, int MODE>
+template <int S, bool MIXED, int MODE, bool PACKED>
 __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned se, float *lds, unsigned lane,
                            double &dacc)
 {
@@ -557,7 +629,7 @@ __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned
         st[k] = a.states[(size_t)v * SP + k];
         acc[k] = (MODE == ITER || MODE == FWD) ? 1.0f : st[k];
     }
-    walk_wide<S, MIXED, MODE>(a, sb, se, lds, lane, dacc, true, acc, n);
+    walk_wide<S, MIXED, MODE, PACKED>(a, sb, se, lds, lane, dacc, true, acc, n);
     if (MODE == ITER || MODE == FWD) {
 #pragma unroll
         for (int k = 0; k < S; ++k)
@@ -567,7 +639,7 @@ __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned
 #pragma unroll
         for (int k = 0; k < S; ++k)
             acc[k] = st[k];
-        walk_wide<S, MIXED, MODE>(a, sb, se, lds, lane, dacc, false, acc, n);
+        walk_wide<S, MIXED, MODE, PACKED>(a, sb, se, lds, lane, dacc, false, acc, n);
     }
     if (MODE == EDGE)
         normalise_positive<S, MIXED>(acc, n);
@@ -580,41 +652,79 @@ template <int S> struct SlotsPerLane {
     static constexpr int value = (S <= 4) ? BP_SLOTS_SMALL : 1;
 };
 
-template <int S, bool MIXED, int MODE>
+/* One chunk: the narrow path, or the one-node path for a node with more
+ * in-slots than a chunk holds. */
+template <int S, bool MIXED, int MODE, int R, bool PACKED>
+__device__ __forceinline__ void run_chunk(const LoopyArgs &a, unsigned c, float *lds, unsigned lane, double &dacc)
+{
+    uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
+    unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
+    if (se - sb <= (unsigned)(WAVE * R))
+        chunk_narrow<S, MIXED, MODE, R, PACKED>(a, nb, ne, sb, se, lds, lane, dacc);
+    else
+        chunk_wide<S, MIXED, MODE, PACKED>(a, nb, sb, se, lds, lane, dacc);
+    wave_sync();
+}
+
+/* per-chunk delta partial (batched networks: chunks never span two
+ * networks, so per-network deltas are sums over chunk ranges) */
+__device__ __forceinline__ void flush_chunk_partial(const LoopyArgs &a, unsigned c, unsigned lane, double &dacc)
+{
+    double d = wave_sum(dacc);
+    if (lane == 0)
+        a.chunk_partials[c] = d;
+    dacc = 0.0;
+}
+
+template <int S, bool MIXED, int MODE, int R = SlotsPerLane<S>::value, bool PACKED = false>
 __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
 {
-    constexpr int R = SlotsPerLane<S>::value;
     if (a.stop && *a.stop)
         return;
     __shared__ float lds[WAVES_PER_BLOCK][S * WAVE * R];
     __shared__ double red[WAVES_PER_BLOCK];
     unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
-    /* wave w of the grid takes the contiguous chunk range
-     * [w * nchunks / W, (w + 1) * nchunks / W), W = waves in the grid; with
-     * the XCD remap each XCD's waves cover one contiguous range */
-    unsigned W = gridDim.x * WAVES_PER_BLOCK;
-    unsigned w = xcd_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wave;
-    unsigned c0 = (unsigned)(((unsigned long long)w * a.nchunks) / W);
-    unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / W);
     double dacc = 0.0;
-    for (unsigned c = c0; c < c1; ++c) {
-        uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
-        unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
-        if (se - sb <= (unsigned)(WAVE * R))
-            chunk_narrow<S, MIXED, MODE, R>(a, nb, ne, sb, se, lds[wave], lane, dacc);
-        else
-            chunk_wide<S, MIXED, MODE>(a, nb, sb, se, lds[wave], lane, dacc);
-        wave_sync();
+    const bool per_chunk = (MODE == ITER || MODE == EDGE) && a.chunk_partials != nullptr;
+    if (a.queue) {
+        /* work queue: each wave takes blocks of qblock consecutive chunks */
+        for (;;) {
+            unsigned base = 0;
+            if (lane == 0)
+                base = atomicAdd(a.queue, a.qblock);
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (base >= a.nchunks)
+                break;
+            unsigned end = base + a.qblock < a.nchunks ? base + a.qblock : a.nchunks;
+            for (unsigned c = base; c < end; ++c) {
+                run_chunk<S, MIXED, MODE, R, PACKED>(a, c, lds[wave], lane, dacc);
+                if (per_chunk)
+                    flush_chunk_partial(a, c, lane, dacc);
+            }
+        }
+    } else {
+        /* wave w of the grid takes the contiguous chunk range
+         * [w * nchunks / W, (w + 1) * nchunks / W), W = waves in the grid;
+         * with the XCD remap each XCD's waves cover one contiguous range */
+        unsigned W = gridDim.x * WAVES_PER_BLOCK;
+        unsigned w = xcd_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wave;
+        unsigned c0 = (unsigned)(((unsigned long long)w * a.nchunks) / W);
+        unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / W);
+        for (unsigned c = c0; c < c1; ++c) {
+            run_chunk<S, MIXED, MODE, R, PACKED>(a, c, lds[wave], lane, dacc);
+            if (per_chunk)
+                flush_chunk_partial(a, c, lane, dacc);
+        }
     }
-    if (MODE == ITER || MODE == EDGE) {
+    if ((MODE == ITER || MODE == EDGE) && !per_chunk) {
         double d = wave_sum(dacc);
         if (lane == 0)
             red[wave] = d;
         __syncthreads();
         if (threadIdx.x == 0) {
             double t = 0.0;
-            for (int w = 0; w < WAVES_PER_BLOCK; ++w)
-                t += red[w];
+            for (int w2 = 0; w2 < WAVES_PER_BLOCK; ++w2)
+                t += red[w2];
             a.partials[blockIdx.x] = t;
         }
     }

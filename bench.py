@@ -54,6 +54,103 @@ def cpu_threads() -> int:
         return os.cpu_count() or 1
 
 
+def bench_adjoint(K: int = 30, c: int = 5, reps: int = 10) -> dict:
+    """The differentiated workload (200k-node 3-state DAG, log-CPT
+    parameters ~N(0,1) softmaxed per parent configuration, 5% evidence,
+    cross-entropy of 1000 query beliefs after K unrolled iterations,
+    checkpoints every c): forward and backward times with HIP events on
+    torch's stream."""
+    import torch
+
+    from belief_network_wrapper import DifferentiableNetwork
+
+    net = DifferentiableNetwork.generate("c5")
+    gen = torch.Generator().manual_seed(6006)
+    theta = torch.randn(net.num_params, generator=gen).cuda()
+    q = torch.randint(0, net.V, (1000,), generator=gen).cuda()
+    t = torch.randint(0, net.S, (1000,), generator=gen).cuda()
+    fw, bw = [], []
+    for rep in range(reps + 2):
+        th = theta.clone().requires_grad_(True)
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e[0].record()
+        b = net.beliefs(net.softmax_tables(th), K, c)
+        loss = -torch.log(torch.clamp(b[q, t], min=1e-30)).mean()
+        e[1].record()
+        e[2].record()
+        loss.backward()
+        e[3].record()
+        torch.cuda.synchronize()
+        if rep >= 2:
+            fw.append(e[0].elapsed_time(e[1]))
+            bw.append(e[2].elapsed_time(e[3]))
+    out = {"workload": f"c5: {net.V} nodes, {net.E} message edges, {net.num_params} parameters, K={K}, "
+                       f"checkpoint every {c}",
+           "forward_ms": float(np.median(fw)), "backward_ms": float(np.median(bw))}
+    out["backward_over_forward"] = out["backward_ms"] / out["forward_ms"]
+    net.close()
+    return out
+
+
+def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: int = 3) -> dict:
+    """The batched workload (512 DAGs, node counts log-uniform in 200-50k,
+    in-degree 0-4, states 2-8, 8 evidence sets of 3% each, 30 iterations):
+    one batch (hipGraph-captured launches, tuned shape) against the plain
+    path run network after network (init + K iterations, each finished
+    before the next starts)."""
+    from workloads import batched_networks
+
+    lib = B.load()
+    t0 = time.perf_counter()
+    batch = B.Batch()
+    plain = []
+    for i, e, g in batched_networks(count=count, evidence_sets=evidence_sets):
+        batch.add(g)
+        plain.append(B.Device(g, 0))
+        lib.graph_destroy(g)
+    build_s = time.perf_counter() - t0
+    batch.upload(0)
+    batch.tune(3, str(ROOT / "profiles" / "batch_shapes.json"))
+    shape = batch.config()
+
+    def step():
+        batch.init()
+        batch.iterate(K)
+
+    step()
+    batch.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    batch.sync()
+    t_batch = (time.perf_counter() - t0) / reps
+    batch.init()
+    kern_ms = batch.profile(5)
+    info = batch.info()
+    for d in plain[:32]:
+        d.init()
+        d.iterate(K)
+        d.sync()
+    t0 = time.perf_counter()
+    for d in plain:
+        d.init()
+        d.iterate(K)
+        d.sync()
+    t_plain = time.perf_counter() - t0
+    out = {"workload": f"{count} networks x {evidence_sets} evidence sets: {info['V']} nodes, {info['E']} message "
+                       f"edges, {K} iterations", "instances": int(info["instances"]),
+           "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3, "speedup": t_plain / t_batch,
+           "shape": {"slots_per_lane": shape[0], "queue_block": shape[1]},
+           "iteration_kernel_ms": kern_ms,
+           "iteration_achieved_GBs": info["alg_bytes"] / (kern_ms * 1e-3) / 1e9,
+           "host_build_s": build_s,
+           "msg_updates_per_s": info["E"] * K / t_batch}
+    for d in plain:
+        d.free()
+    batch.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +158,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--preset", default="c2b")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--forms", default="adjoint,batched",
+                    help="comma list of the further forms measured in the same run (adjoint, batched; '' for none)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -148,6 +247,13 @@ def main():
                          f"({t_cpu:.2f} s wall, {threads} threads)"}
         lib.graph_destroy(h)
 
+    forms = {}
+    wanted = [f for f in args.forms.split(",") if f]
+    if "adjoint" in wanted:
+        forms["adjoint"] = bench_adjoint()
+    if "batched" in wanted:
+        forms["batched"] = bench_batched()
+
     out = {
         "metric": "edge message-updates/s at 1M edges (loopy BP, 20 iterations + FP64 delta)",
         "value": value,
@@ -171,6 +277,7 @@ def main():
         "cpu_baseline": cpu,
         "parity_vs_cpu": parity,
         "final_delta": float(deltas[-1]),
+        "forms": forms,
     }
     dev.free()
     lib.graph_destroy(g)

@@ -1,0 +1,132 @@
+"""The batched form on the GPU (include/bp_batch.h): every instance's
+beliefs, messages and deltas equal running it alone through the plain path
+(bp_upload + bp_loopy_init + bp_loopy_iterate), for every kernel shape the
+tuner can pick; and an instance equals the oracle."""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+import bp_ctypes as B
+from conftest import same_bits, bits_report
+from workloads import batched_networks
+
+pytestmark = pytest.mark.gpu
+
+
+def build(count, evidence_sets, lo, hi, preset_graphs=None):
+    b = B.Batch()
+    graphs = []
+    for i, e, g in batched_networks(count=count, evidence_sets=evidence_sets, lo=lo, hi=hi):
+        b.add(g)
+        graphs.append(g)
+    return b, graphs
+
+
+def plain_run(g, n):
+    d = B.Device(g, 0)
+    d.init()
+    deltas = d.iterate(n, want_deltas=True)
+    d.download(g)
+    d.free()
+    return deltas
+
+
+def copy_graph_results(lib, g):
+    return B.states(g).copy(), B.messages(g, "prev").copy(), B.messages(g, "cur").copy()
+
+
+def test_batch_matches_plain_per_instance(lib):
+    n = 30
+    b, graphs = build(10, 3, 200, 3000)
+    print(f"batch: {b.info()}", flush=True)
+    b.upload(0)
+    b.init()
+    bd = b.iterate(n, want_deltas=True)
+    for i, g in enumerate(graphs):
+        b.download(i, g)
+    batch_res = [copy_graph_results(lib, g) for g in graphs]
+    for g in graphs:
+        lib.graph_destroy(g)
+    # the same instances, one at a time
+    for i, e, g in batched_networks(count=10, evidence_sets=3, lo=200, hi=3000):
+        k = i * 3 + e
+        pd = plain_run(g, n)
+        st, mp, mc = copy_graph_results(lib, g)
+        assert same_bits(batch_res[k][0], st), f"instance {k}: {bits_report(batch_res[k][0], st)}"
+        assert same_bits(batch_res[k][1], mp) and same_bits(batch_res[k][2], mc), f"instance {k} messages"
+        assert np.allclose(bd[:, k], pd, rtol=1e-12, atol=1e-300), f"instance {k} deltas"
+        lib.graph_destroy(g)
+    b.free()
+
+
+@pytest.mark.parametrize("preset_hi", [4, 8])
+def test_every_shape_gives_the_same_bits(lib, preset_hi):
+    b = B.Batch()
+    gs = []
+    for s in range(6):
+        bn = B.generate("c4net", num_nodes=400 + 300 * s, states_hi=preset_hi, seed=900 + s)
+        g = B.bn_to_graph(bn)
+        lib.bp_bn_free(bn)
+        b.add(g)
+        gs.append(g)
+    b.upload(0)
+    ref = None
+    shapes = [(1, 0), (1, 1), (1, 4), (1, 16)] + ([(2, 0), (2, 4)] if preset_hi <= 6 else [])
+    for R, q in shapes:
+        b.set_config(R, q)
+        b.init()
+        d = b.iterate(12, want_deltas=True)
+        b.download(len(gs) - 1, gs[-1])
+        got = (B.states(gs[-1]).copy(), d)
+        if ref is None:
+            ref = got
+        else:
+            assert same_bits(got[0], ref[0]), f"shape {R},{q}"
+            assert np.array_equal(got[1], ref[1]), f"shape {R},{q} deltas"
+    b.free()
+    for g in gs:
+        lib.graph_destroy(g)
+
+
+def test_tuner_picks_and_caches_a_shape(lib, tmp_path):
+    b, graphs = build(6, 2, 300, 2000)
+    b.upload(0)
+    cache = tmp_path / "tune.json"
+    b.tune(3, str(cache))
+    first = b.config()
+    data = json.loads(cache.read_text())
+    assert len(data) == 1
+    (entry,) = data.values()
+    assert (entry["slots_per_lane"], entry["queue_block"]) == first
+    b.set_config(1, 16 if first != (1, 16) else 0)
+    b.tune(3, str(cache))  # a hit: no timing, same choice
+    assert b.config() == first
+    b.init()
+    b.iterate(5)
+    b.sync()
+    b.free()
+    for g in graphs:
+        lib.graph_destroy(g)
+
+
+def test_instance_matches_oracle(lib, orc):
+    b, graphs = build(3, 2, 500, 1500)
+    b.upload(0)
+    b.init()
+    b.iterate(20)
+    k = 4
+    b.download(k, graphs[k])
+    got = B.states(graphs[k]).copy()
+    for i, e, g in batched_networks(count=3, evidence_sets=2, lo=500, hi=1500):
+        if i * 2 + e == k:
+            orc.init_previous_edge(g)
+            for _ in range(20):
+                orc.loopy_iteration(g)
+            assert same_bits(got, B.states(g)), bits_report(got, B.states(g))
+        lib.graph_destroy(g)
+    b.free()
+    for g in graphs:
+        lib.graph_destroy(g)

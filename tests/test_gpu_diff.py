@@ -109,10 +109,16 @@ def test_gradient_matches_fp64_oracle(lib, K):
     l64 = -torch.log(torch.clamp(b64[q, t], min=FLOOR)).mean()
     l64.backward()
     g64 = p64.grad.numpy()
+    assert abs(float(loss.detach()) - float(l64.detach())) <= 1e-4 * max(1.0, abs(float(l64.detach())))
+    if K == 0:
+        # no iteration: the unobserved query nodes keep their default
+        # belief (1.0 per state), which no parameter reaches
+        assert not g64.any() and not g32.any()
+        teardown(lib, bn, g, net)
+        return
     rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
     cos = g32 @ g64 / (np.linalg.norm(g32) * np.linalg.norm(g64))
     print(f"K={K}: loss {float(loss):.6f} vs {float(l64):.6f}; gradient rel l2 {rel:.3e}, cosine {cos:.12f}")
-    assert abs(float(loss) - float(l64)) <= 1e-4 * max(1.0, abs(float(l64)))
     assert rel < 2e-3
     assert cos > 1 - 1e-6
     teardown(lib, bn, g, net)
@@ -195,8 +201,8 @@ def test_full_scale_c5(lib):
             return float(-torch.log(torch.clamp(b[q, t], min=FLOOR)).mean())
 
     l64 = loss64(th64)
-    print(f"loss f32 {float(loss):.6f} fp64 {l64:.6f}", flush=True)
-    assert abs(float(loss) - l64) <= 1e-3 * max(1.0, abs(l64))
+    print(f"loss f32 {float(loss.detach()):.6f} fp64 {l64:.6f}", flush=True)
+    assert abs(float(loss.detach()) - l64) <= 1e-3 * max(1.0, abs(l64))
     rng = np.random.default_rng(1)
     for _ in range(2):
         v = torch.tensor(rng.standard_normal(net.num_params))
