@@ -182,6 +182,23 @@ class DifferentiableNetwork:
         """[V, S] beliefs after K iterations; differentiable in `tables`."""
         return UnrolledLoopyBP.apply(tables, self, K, checkpoint_every)[:, : self.S]
 
+    def beliefs_from_log(self, theta, K: int, checkpoint_every: int = 5):
+        """[V, S] beliefs after K iterations with tables = softmax_tables(theta),
+        the softmax fused into the library; differentiable in `theta`."""
+        return UnrolledLoopyBPLog.apply(theta, self, K, checkpoint_every)[:, : self.S]
+
+    def _set_log_params(self, theta):
+        B.check(self.lib.bp_diff_set_log_params(self.h, C.c_void_p(theta.data_ptr()), _stream_ptr()),
+                "bp_diff_set_log_params")
+
+    def _backward_log(self, grad_beliefs):
+        import torch
+
+        g = torch.empty(self.num_params, dtype=torch.float32, device=self.device)
+        B.check(self.lib.bp_diff_backward_log(self.h, C.c_void_p(grad_beliefs.data_ptr()), C.c_void_p(g.data_ptr()),
+                                              _stream_ptr()), "bp_diff_backward_log")
+        return g
+
     # raw handle calls (device pointers on torch's current stream)
     def _set_params(self, tables):
         B.check(self.lib.bp_diff_set_params(self.h, C.c_void_p(tables.data_ptr()), _stream_ptr()),
@@ -232,7 +249,7 @@ class DifferentiableNetwork:
             pass
 
 
-def _function():
+def _functions():
     import torch
 
     class UnrolledLoopyBP(torch.autograd.Function):
@@ -263,20 +280,46 @@ def _function():
             g, _, _ = net._backward(grad.contiguous())
             return g, None, None, None
 
-    return UnrolledLoopyBP
+    class UnrolledLoopyBPLog(torch.autograd.Function):
+        """beliefs[V, stride] = K unrolled loopy iterations of
+        tables = softmax(theta) per row, the softmax fused into the library;
+        backward: bp_diff_backward_log."""
+
+        @staticmethod
+        def forward(ctx, theta, net, K, c):
+            if theta.dtype != torch.float32 or not theta.is_contiguous() or theta.numel() != net.num_params:
+                raise ValueError("theta must be a contiguous float32 vector of num_params entries")
+            net._set_log_params(theta)
+            out = net._forward(int(K), int(c))
+            ctx.net = net
+            ctx.generation = net._generation
+            return out
+
+        @staticmethod
+        def backward(ctx, grad):
+            net = ctx.net
+            if net._generation != ctx.generation:
+                raise RuntimeError("another forward ran on this network before the backward")
+            return net._backward_log(grad.contiguous()), None, None, None
+
+    return {"UnrolledLoopyBP": UnrolledLoopyBP, "UnrolledLoopyBPLog": UnrolledLoopyBPLog}
 
 
 class _Lazy:
     """torch is imported only when the differentiable form is used."""
 
-    _cls = None
+    _classes = None
 
-    def __getattr__(self, name):
-        if _Lazy._cls is None:
-            _Lazy._cls = _function()
-        return getattr(_Lazy._cls, name)
+    def __init__(self, name):
+        self._name = name
+
+    def __getattr__(self, attr):
+        if _Lazy._classes is None:
+            _Lazy._classes = _functions()
+        return getattr(_Lazy._classes[self._name], attr)
 
 
-UnrolledLoopyBP = _Lazy()
+UnrolledLoopyBP = _Lazy("UnrolledLoopyBP")
+UnrolledLoopyBPLog = _Lazy("UnrolledLoopyBPLog")
 
-__all__ = ["BeliefNetwork", "DifferentiableNetwork", "UnrolledLoopyBP", "C"]
+__all__ = ["BeliefNetwork", "DifferentiableNetwork", "UnrolledLoopyBP", "UnrolledLoopyBPLog", "C"]

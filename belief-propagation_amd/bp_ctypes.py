@@ -202,13 +202,15 @@ _SIGNATURES = {
     "bp_diff_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bp_diff_backward_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "bp_diff_tables": (C.c_int, [C.c_void_p, f32p]),
+    "bp_diff_set_log_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "bp_diff_backward_log": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     # bp_batch.h
     "bp_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "bp_batch_free": (None, [C.c_void_p]),
     "bp_batch_add": (C.c_int, [C.c_void_p, GraphP, C.POINTER(C.c_uint)]),
     "bp_batch_upload": (C.c_int, [C.c_void_p, C.c_int]),
-    "bp_batch_set_config": (C.c_int, [C.c_void_p, C.c_uint, C.c_uint]),
-    "bp_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
+    "bp_batch_set_config": (C.c_int, [C.c_void_p, C.c_uint, C.c_uint, C.c_uint]),
+    "bp_batch_get_config": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
     "bp_batch_tune": (C.c_int, [C.c_void_p, C.c_uint, C.c_char_p]),
     "bp_batch_init": (C.c_int, [C.c_void_p]),
     "bp_batch_iterate": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_double)]),
@@ -388,13 +390,15 @@ class Batch:
     def upload(self, device: int = -1):
         check(self.lib.bp_batch_upload(self.h, device), "bp_batch_upload")
 
-    def config(self) -> tuple[int, int]:
-        r, q = C.c_uint(), C.c_uint()
-        check(self.lib.bp_batch_get_config(self.h, C.byref(r), C.byref(q)), "bp_batch_get_config")
-        return r.value, q.value
+    def config(self) -> tuple[int, int, int]:
+        """(slots per lane, lanes per slot, queue block)"""
+        r, l, q = C.c_uint(), C.c_uint(), C.c_uint()
+        check(self.lib.bp_batch_get_config(self.h, C.byref(r), C.byref(l), C.byref(q)), "bp_batch_get_config")
+        return r.value, l.value, q.value
 
-    def set_config(self, slots_per_lane: int, queue_block: int):
-        check(self.lib.bp_batch_set_config(self.h, slots_per_lane, queue_block), "bp_batch_set_config")
+    def set_config(self, slots_per_lane: int, lanes_per_slot: int, queue_block: int):
+        check(self.lib.bp_batch_set_config(self.h, slots_per_lane, lanes_per_slot, queue_block),
+              "bp_batch_set_config")
 
     def tune(self, iterations: int = 4, cache: str | None = None):
         check(self.lib.bp_batch_tune(self.h, iterations, cache.encode() if cache else None), "bp_batch_tune")

@@ -130,7 +130,7 @@ struct bp_batch {
     Shape shapes[2];
 
     /* configuration and run state */
-    unsigned R = 1, qblock = 0;
+    unsigned R = 1, L = 1, qblock = 0;
     int latest = 0, bcur = 0;
     unsigned swaps = 0;
     bool ready = false;
@@ -415,36 +415,43 @@ extern "C" int bp_batch_upload(bp_batch *b, int device)
     return 0;
 }
 
-static bool valid_config(const bp_batch *b, unsigned R, unsigned qblock)
+static bool valid_config(const bp_batch *b, unsigned R, unsigned L, unsigned qblock)
 {
     (void)qblock;
-    /* two slots per lane would spill registers beyond six states */
-    return (R == 1) || (R == 2 && b->S <= 6);
+    if (L == 1) /* two slots per lane would spill registers beyond six states */
+        return (R == 1) || (R == 2 && b->S <= 6);
+    /* several lanes per slot: one slot per lane-group, four or more states */
+    return R == 1 && b->S >= 4 && (L == 2 || L == 4 || L == 8);
 }
 
-extern "C" int bp_batch_set_config(bp_batch *b, unsigned int slots_per_lane, unsigned int queue_block)
+extern "C" int bp_batch_set_config(bp_batch *b, unsigned int slots_per_lane, unsigned int lanes_per_slot,
+                                   unsigned int queue_block)
 {
     bp_clear_error();
     if (!b || !b->uploaded) {
         bp_set_error("bp_batch_set_config: batch not uploaded");
         return -1;
     }
-    if (!valid_config(b, slots_per_lane, queue_block)) {
-        bp_set_error("bp_batch_set_config: %u slots per lane is not available for %u states", slots_per_lane, b->S);
+    if (!valid_config(b, slots_per_lane, lanes_per_slot, queue_block)) {
+        bp_set_error("bp_batch_set_config: %u slots per lane with %u lanes per slot is not available for %u states",
+                     slots_per_lane, lanes_per_slot, b->S);
         return -1;
     }
     b->R = slots_per_lane;
+    b->L = lanes_per_slot;
     b->qblock = queue_block;
     return 0;
 }
 
-extern "C" int bp_batch_get_config(const bp_batch *b, unsigned int *slots_per_lane, unsigned int *queue_block)
+extern "C" int bp_batch_get_config(const bp_batch *b, unsigned int *slots_per_lane, unsigned int *lanes_per_slot,
+                                   unsigned int *queue_block)
 {
-    if (!b || !slots_per_lane || !queue_block) {
+    if (!b || !slots_per_lane || !lanes_per_slot || !queue_block) {
         bp_set_error("bp_batch_get_config: NULL argument");
         return -1;
     }
     *slots_per_lane = b->R;
+    *lanes_per_slot = b->L;
     *queue_block = b->qblock;
     return 0;
 }
@@ -488,30 +495,45 @@ static unsigned batch_grid(const bp_batch *b)
     return G;
 }
 
-template <int S, int R>
-static void launch_batch_sr(int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
+template <int S, int R, int L>
+static void launch_batch_srl(int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
 {
     dim3 grid(G), block(BLOCK);
     if (mode == ITER) {
         if (ev)
-            hipExtLaunchKernelGGL((k_loopy<S, true, ITER, R, true>), grid, block, 0, st, ev[0], ev[1], 0, a);
+            hipExtLaunchKernelGGL((k_loopy<S, true, ITER, R, true, L>), grid, block, 0, st, ev[0], ev[1], 0, a);
         else
-            hipLaunchKernelGGL((k_loopy<S, true, ITER, R, true>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_loopy<S, true, ITER, R, true, L>), grid, block, 0, st, a);
     } else {
-        hipLaunchKernelGGL((k_loopy<S, true, INIT, R, true>), grid, block, 0, st, a);
+        hipLaunchKernelGGL((k_loopy<S, true, INIT, R, true, L>), grid, block, 0, st, a);
     }
 }
 
 template <int S>
-static void launch_batch_s(unsigned R, int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
+static void launch_batch_s(unsigned R, unsigned L, int mode, const LoopyArgs &a, unsigned G, hipStream_t st,
+                           const hipEvent_t *ev)
 {
     if constexpr (S <= 6) {
         if (R == 2) {
-            launch_batch_sr<S, 2>(mode, a, G, st, ev);
+            launch_batch_srl<S, 2, 1>(mode, a, G, st, ev);
             return;
         }
     }
-    launch_batch_sr<S, 1>(mode, a, G, st, ev);
+    if constexpr (S >= 4) {
+        if (L == 2) {
+            launch_batch_srl<S, 1, 2>(mode, a, G, st, ev);
+            return;
+        }
+        if (L == 4) {
+            launch_batch_srl<S, 1, 4>(mode, a, G, st, ev);
+            return;
+        }
+        if (L == 8) {
+            launch_batch_srl<S, 1, 8>(mode, a, G, st, ev);
+            return;
+        }
+    }
+    launch_batch_srl<S, 1, 1>(mode, a, G, st, ev);
 }
 
 static void launch_batch(bp_batch *b, int mode, const LoopyArgs &a, hipStream_t st, const hipEvent_t *ev = nullptr)
@@ -522,16 +544,16 @@ static void launch_batch(bp_batch *b, int mode, const LoopyArgs &a, hipStream_t 
         hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, b->d_queue);
     unsigned G = batch_grid(b);
     switch (b->S) {
-    case 1: launch_batch_s<1>(b->R, mode, a, G, st, ev); break;
-    case 2: launch_batch_s<2>(b->R, mode, a, G, st, ev); break;
-    case 3: launch_batch_s<3>(b->R, mode, a, G, st, ev); break;
-    case 4: launch_batch_s<4>(b->R, mode, a, G, st, ev); break;
-    case 5: launch_batch_s<5>(b->R, mode, a, G, st, ev); break;
-    case 6: launch_batch_s<6>(b->R, mode, a, G, st, ev); break;
-    case 7: launch_batch_s<7>(b->R, mode, a, G, st, ev); break;
-    case 8: launch_batch_s<8>(b->R, mode, a, G, st, ev); break;
-    case 9: launch_batch_s<9>(b->R, mode, a, G, st, ev); break;
-    default: launch_batch_s<10>(b->R, mode, a, G, st, ev); break;
+    case 1: launch_batch_s<1>(b->R, b->L, mode, a, G, st, ev); break;
+    case 2: launch_batch_s<2>(b->R, b->L, mode, a, G, st, ev); break;
+    case 3: launch_batch_s<3>(b->R, b->L, mode, a, G, st, ev); break;
+    case 4: launch_batch_s<4>(b->R, b->L, mode, a, G, st, ev); break;
+    case 5: launch_batch_s<5>(b->R, b->L, mode, a, G, st, ev); break;
+    case 6: launch_batch_s<6>(b->R, b->L, mode, a, G, st, ev); break;
+    case 7: launch_batch_s<7>(b->R, b->L, mode, a, G, st, ev); break;
+    case 8: launch_batch_s<8>(b->R, b->L, mode, a, G, st, ev); break;
+    case 9: launch_batch_s<9>(b->R, b->L, mode, a, G, st, ev); break;
+    default: launch_batch_s<10>(b->R, b->L, mode, a, G, st, ev); break;
     }
 }
 
@@ -586,7 +608,7 @@ extern "C" int bp_batch_iterate(bp_batch *b, unsigned int n, double *deltas)
     if (ensure_deltas(b, n))
         return -1;
     char key[96];
-    snprintf(key, sizeof(key), "%u/%u/%u/%d/%d", n, b->R, b->qblock, b->latest, b->bcur);
+    snprintf(key, sizeof(key), "%u/%u/%u/%u/%d/%d", n, b->R, b->L, b->qblock, b->latest, b->bcur);
     auto it = b->graphs.find(key);
     if (it == b->graphs.end()) {
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -751,7 +773,7 @@ static std::string batch_signature(const bp_batch *b)
     return buf;
 }
 
-static bool cache_lookup(const char *path, const std::string &sig, unsigned *R, unsigned *q)
+static bool cache_lookup(const char *path, const std::string &sig, unsigned *R, unsigned *L, unsigned *q)
 {
     if (!path)
         return false;
@@ -768,15 +790,17 @@ static bool cache_lookup(const char *path, const std::string &sig, unsigned *R, 
     size_t pos = text.find(key);
     if (pos == std::string::npos)
         return false;
-    size_t r = text.find("\"slots_per_lane\":", pos), qq = text.find("\"queue_block\":", pos);
-    if (r == std::string::npos || qq == std::string::npos)
+    size_t r = text.find("\"slots_per_lane\":", pos), l = text.find("\"lanes_per_slot\":", pos),
+           qq = text.find("\"queue_block\":", pos);
+    if (r == std::string::npos || l == std::string::npos || qq == std::string::npos)
         return false;
     *R = (unsigned)strtoul(text.c_str() + r + 17, nullptr, 10);
+    *L = (unsigned)strtoul(text.c_str() + l + 17, nullptr, 10);
     *q = (unsigned)strtoul(text.c_str() + qq + 14, nullptr, 10);
     return true;
 }
 
-static void cache_store(const char *path, const std::string &sig, unsigned R, unsigned q, double ms)
+static void cache_store(const char *path, const std::string &sig, unsigned R, unsigned L, unsigned q, double ms)
 {
     if (!path)
         return;
@@ -790,8 +814,10 @@ static void cache_store(const char *path, const std::string &sig, unsigned R, un
     }
     size_t end = text.rfind('}');
     char entry[512];
-    snprintf(entry, sizeof(entry), "  \"%s\": {\"slots_per_lane\": %u, \"queue_block\": %u, \"ms_per_iteration\": %.6f}",
-             sig.c_str(), R, q, ms);
+    snprintf(entry, sizeof(entry),
+             "  \"%s\": {\"slots_per_lane\": %u, \"lanes_per_slot\": %u, \"queue_block\": %u, "
+             "\"ms_per_iteration\": %.6f}",
+             sig.c_str(), R, L, q, ms);
     std::string out;
     if (end == std::string::npos || text.find('"') == std::string::npos)
         out = std::string("{\n") + entry + "\n}\n";
@@ -816,37 +842,43 @@ extern "C" int bp_batch_tune(bp_batch *b, unsigned int iterations, const char *c
     }
     HIP_TRY(hipSetDevice(b->device));
     std::string sig = batch_signature(b);
-    unsigned R = 0, q = 0;
-    if (cache_lookup(cache_path, sig, &R, &q) && valid_config(b, R, q)) {
+    unsigned R = 0, L = 0, q = 0;
+    if (cache_lookup(cache_path, sig, &R, &L, &q) && valid_config(b, R, L, q)) {
         b->R = R;
+        b->L = L;
         b->qblock = q;
         return 0;
     }
     const unsigned Rs[2] = {1, 2};
+    const unsigned Ls[4] = {1, 2, 4, 8};
     const unsigned Qs[4] = {0, 1, 4, 16};
     double best = 1e300;
-    unsigned bestR = b->R, bestQ = b->qblock;
+    unsigned bestR = b->R, bestL = b->L, bestQ = b->qblock;
     unsigned n = iterations ? iterations : 4;
     for (unsigned R2 : Rs)
-        for (unsigned q2 : Qs) {
-            if (!valid_config(b, R2, q2))
-                continue;
-            b->R = R2;
-            b->qblock = q2;
-            if (bp_batch_init(b))
-                return -1;
-            float ms = 0.0f;
-            if (bp_batch_profile(b, 1, &ms) || bp_batch_profile(b, n, &ms))
-                return -1;
-            if (ms < best) {
-                best = ms;
-                bestR = R2;
-                bestQ = q2;
+        for (unsigned L2 : Ls)
+            for (unsigned q2 : Qs) {
+                if (!valid_config(b, R2, L2, q2))
+                    continue;
+                b->R = R2;
+                b->L = L2;
+                b->qblock = q2;
+                if (bp_batch_init(b))
+                    return -1;
+                float ms = 0.0f;
+                if (bp_batch_profile(b, 1, &ms) || bp_batch_profile(b, n, &ms))
+                    return -1;
+                if (ms < best) {
+                    best = ms;
+                    bestR = R2;
+                    bestL = L2;
+                    bestQ = q2;
+                }
             }
-        }
     b->R = bestR;
+    b->L = bestL;
     b->qblock = bestQ;
     b->ready = false; /* the timing runs changed the state: init again */
-    cache_store(cache_path, sig, bestR, bestQ, best);
+    cache_store(cache_path, sig, bestR, bestL, bestQ, best);
     return 0;
 }

@@ -50,14 +50,6 @@ namespace {
 
 constexpr unsigned NONE = 0xffffffffu;
 
-/* table of one slot: the parameters of its definition and the parent */
-struct JMap {
-    unsigned tb;     /* first parameter of the definition */
-    unsigned slice;  /* TABLE entries per child state */
-    unsigned offset; /* product of the state counts of the later GIVENs */
-    unsigned meta;   /* parent states | child states << 8 | reverse << 16 */
-};
-
 struct ParentRec {
     unsigned offset, n_par, slot_f, slot_r;
 };
@@ -85,32 +77,34 @@ __global__ void k_copy(const float *__restrict__ src, float *__restrict__ dst, s
         dst[i] = src[i];
 }
 
-__global__ void k_tables(const float *__restrict__ P, const JMap *__restrict__ map, float *__restrict__ J,
-                         unsigned E, unsigned S)
+/* One thread per (parent record, child state k): J_fwd[j][k] for every
+ * parent state j, each the sequential f32 sum over the parent's blocks in
+ * ascending TABLE order -- the reader's loop -- written to the forward slot
+ * and, transposed, to the reverse slot. */
+__global__ void k_tables(const float *__restrict__ P, const ParentRec *__restrict__ parents,
+                         const unsigned *__restrict__ par_def, const unsigned *__restrict__ def_tb,
+                         const unsigned *__restrict__ def_slice, float *__restrict__ J, unsigned nrec, unsigned S)
 {
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrec * S)
+        return;
+    unsigned pr = t / S, k = t % S;
+    ParentRec rec = parents[pr];
+    unsigned d = par_def[pr];
+    unsigned slice = def_slice[d], offset = rec.offset, npar = rec.n_par, stride = npar * offset;
+    const float *T = P + def_tb[d] + (size_t)k * slice;
     size_t SS = (size_t)S * S;
-    if (t >= (size_t)E * SS)
-        return;
-    unsigned s = (unsigned)(t / SS), ij = (unsigned)(t % SS), i = ij / S, j = ij % S;
-    JMap m = map[s];
-    unsigned npar = m.meta & 255u, nch = (m.meta >> 8) & 255u, rev = m.meta >> 16;
-    unsigned jp = rev ? j : i, k = rev ? i : j;
-    if (jp >= npar || k >= nch) {
-        J[t] = 0.0f;
-        return;
+    for (unsigned j = 0; j < npar; ++j) {
+        float acc = 0.0f;
+        for (unsigned b = 0; j * offset + b * stride < slice; ++b) {
+            const float *blk = T + j * offset + b * stride;
+            for (unsigned q = 0; q < offset; ++q)
+                acc += blk[q];
+        }
+        J[rec.slot_f * SS + j * S + k] = acc;
+        if (rec.slot_r != NONE)
+            J[rec.slot_r * SS + k * S + j] = acc;
     }
-    unsigned long long offset = m.offset, stride = (unsigned long long)npar * offset, slice = m.slice;
-    const float *T = P + m.tb + (unsigned long long)k * slice;
-    float acc = 0.0f;
-    for (unsigned long long b = 0;; ++b) {
-        unsigned long long start = jp * offset + b * stride;
-        if (b == 0 ? start > slice : start + offset > slice)
-            break;
-        for (unsigned long long idx = start; idx < start + offset; ++idx)
-            acc += T[idx];
-    }
-    J[t] = acc;
 }
 
 __global__ void k_initial(const float *__restrict__ P, const unsigned *__restrict__ prior_tb,
@@ -362,6 +356,43 @@ __global__ void k_param_grad(const float *__restrict__ jbar, const float *__rest
     out[p] = g;
 }
 
+/* P = softmax of theta over each row (a definition's child states under
+ * one parent configuration): entries first[r] + k * stride[r], k < S */
+__global__ void k_softmax(const float *__restrict__ theta, const unsigned *__restrict__ first,
+                          const unsigned *__restrict__ stride, float *__restrict__ P, unsigned nrows, unsigned S)
+{
+    unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows)
+        return;
+    unsigned f = first[r], st = stride[r];
+    float mx = -INFINITY;
+    for (unsigned k = 0; k < S; ++k)
+        mx = fmaxf(mx, theta[f + k * st]);
+    float e[MAX_STATES], sum = 0.0f;
+    for (unsigned k = 0; k < S; ++k) {
+        e[k] = expf(theta[f + k * st] - mx);
+        sum += e[k];
+    }
+    for (unsigned k = 0; k < S; ++k)
+        P[f + k * st] = e[k] / sum;
+}
+
+/* dtheta = P (.) (dP - <P, dP>) per row */
+__global__ void k_softmax_back(const float *__restrict__ P, const float *__restrict__ gP,
+                               const unsigned *__restrict__ first, const unsigned *__restrict__ stride,
+                               float *__restrict__ gtheta, unsigned nrows, unsigned S)
+{
+    unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows)
+        return;
+    unsigned f = first[r], st = stride[r];
+    float dot = 0.0f;
+    for (unsigned k = 0; k < S; ++k)
+        dot += P[f + k * st] * gP[f + k * st];
+    for (unsigned k = 0; k < S; ++k)
+        gtheta[f + k * st] = P[f + k * st] * (gP[f + k * st] - dot);
+}
+
 /* slot order -> reference edge order */
 __global__ void k_slot_to_edge(const float *__restrict__ src, const unsigned *__restrict__ slot_edge,
                                float *__restrict__ dst, unsigned E, unsigned width)
@@ -384,13 +415,15 @@ struct bp_diff {
     std::vector<uint64_t> row_first;
     std::vector<uint32_t> row_stride, row_len;
 
-    JMap *jmap = nullptr;
+    unsigned *par_def = nullptr; /* definition of each parent record */
+    unsigned nrec = 0;
     unsigned *prior_tb = nullptr;
     unsigned *out_off = nullptr, *out_slot = nullptr, *slot_edge = nullptr;
     unsigned *def_of = nullptr, *def_tb = nullptr, *def_slice = nullptr, *def_pbegin = nullptr, *def_prior = nullptr;
     ParentRec *parents = nullptr;
 
     float *P = nullptr;     /* parameters */
+    unsigned *row_first_d = nullptr, *row_stride_d = nullptr; /* softmax rows */
     float *st0 = nullptr;   /* initial beliefs */
     float *gP = nullptr;    /* dL/dparams */
     float *gin = nullptr;   /* dL/dbeliefs (copy) */
@@ -437,7 +470,7 @@ extern "C" void bp_diff_free(bp_diff *x)
         (void)hipDeviceSynchronize();
     }
     diff_release_storage(x);
-    void *ps[] = {x->jmap, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
+    void *ps[] = {x->row_first_d, x->row_stride_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
                   x->def_pbegin, x->def_prior, x->parents, x->P, x->st0, x->gP, x->gin, x->gst[0], x->gst[1],
                   x->w[0], x->w[1], x->jbar, x->dst0p, x->dst0};
     for (void *p : ps)
@@ -511,9 +544,9 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
     /* replay the reader's edge emission: definitions in order, parents last
      * GIVEN first, forward edge then reverse edge unless the parent is
      * observed (bp_graph_add_cpt_edges) */
-    std::vector<JMap> jm(E);
     std::vector<unsigned> def_of(x->num_params), def_tb(npd), def_slice(npd), def_pbegin(npd + 1, 0), def_prior(npd, NONE);
     std::vector<ParentRec> prs;
+    std::vector<unsigned> par_def;
     unsigned e = 0;
     for (unsigned dd = 0; dd < bn->num_defs; ++dd) {
         uint64_t g0 = bn->def_given_begin[dd], g1 = bn->def_given_begin[dd + 1];
@@ -537,7 +570,6 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
                 return -1;
             }
             pr.slot_f = d->edge_slot[e];
-            jm[pr.slot_f] = JMap{(unsigned)t0, slice, offset, npar | (nch << 8)};
             ++e;
             if (g->observed_nodes[parent] != 1) {
                 if (e >= E || g->edges_src_index[e] != child || g->edges_dest_index[e] != parent) {
@@ -545,10 +577,10 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
                     return -1;
                 }
                 pr.slot_r = d->edge_slot[e];
-                jm[pr.slot_r] = JMap{(unsigned)t0, slice, offset, npar | (nch << 8) | (1u << 16)};
                 ++e;
             }
             prs.push_back(pr);
+            par_def.push_back(dd);
             offset *= npar;
         }
         def_pbegin[dd + 1] = (unsigned)prs.size();
@@ -580,6 +612,12 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
         }
     }
     x->num_rows = (unsigned)x->row_first.size();
+    {
+        std::vector<unsigned> rf(x->row_first.begin(), x->row_first.end());
+        if (upload_vec(&x->row_first_d, rf) || upload_vec(&x->row_stride_d, x->row_stride))
+            return -1;
+    }
+    x->nrec = (unsigned)prs.size();
     /* source-grouped slots */
     std::vector<unsigned> out_off(V + 1, 0), out_slot(E), slot_src(E);
     for (unsigned s = 0; s < E; ++s) {
@@ -594,11 +632,11 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
             out_slot[fill[slot_src[s]]++] = s;
     }
     size_t VS = (size_t)V * d->SP, ES = (size_t)E * d->SP;
-    if (upload_vec(&x->jmap, jm) || upload_vec(&x->prior_tb, prior_tb) || upload_vec(&x->out_off, out_off) ||
+    if (upload_vec(&x->prior_tb, prior_tb) || upload_vec(&x->out_off, out_off) ||
         upload_vec(&x->out_slot, out_slot) || upload_vec(&x->slot_edge, d->slot_edge) ||
         upload_vec(&x->def_of, def_of) || upload_vec(&x->def_tb, def_tb) || upload_vec(&x->def_slice, def_slice) ||
         upload_vec(&x->def_pbegin, def_pbegin) || upload_vec(&x->def_prior, def_prior) ||
-        upload_vec(&x->parents, prs) || dmalloc(&x->P, x->num_params) || dmalloc(&x->gP, x->num_params) ||
+        upload_vec(&x->parents, prs) || upload_vec(&x->par_def, par_def) || dmalloc(&x->P, x->num_params) || dmalloc(&x->gP, x->num_params) ||
         dmalloc(&x->st0, VS) || dmalloc(&x->gin, VS) || dmalloc(&x->gst[0], VS) || dmalloc(&x->gst[1], VS) ||
         dmalloc(&x->w[0], ES) || dmalloc(&x->w[1], ES) || dmalloc(&x->jbar, (size_t)E * S * S) ||
         dmalloc(&x->dst0p, VS) || dmalloc(&x->dst0, VS))
@@ -691,9 +729,36 @@ extern "C" int bp_diff_set_params(bp_diff *x, const float *params, void *stream)
     if (x->num_params)
         hipLaunchKernelGGL(k_copy, dim3(blocks_for(x->num_params)), dim3(256), 0, st, params, x->P,
                            (size_t)x->num_params);
-    size_t nJ = (size_t)x->E * x->S * x->S;
-    if (nJ)
-        hipLaunchKernelGGL(k_tables, dim3(blocks_for(nJ)), dim3(256), 0, st, x->P, x->jmap, x->d->J, x->E, x->S);
+    size_t nT = (size_t)x->nrec * x->S;
+    if (nT)
+        hipLaunchKernelGGL(k_tables, dim3(blocks_for(nT)), dim3(256), 0, st, x->P, x->parents, x->par_def, x->def_tb,
+                           x->def_slice, x->d->J, x->nrec, x->S);
+    size_t VS = (size_t)x->V * x->SP;
+    if (VS)
+        hipLaunchKernelGGL(k_initial, dim3(blocks_for(VS)), dim3(256), 0, st, x->P, x->prior_tb, x->d->states0,
+                           x->st0, x->V, x->S, x->SP);
+    HIP_TRY(hipGetLastError());
+    x->have_params = true;
+    x->have_forward = false;
+    return 0;
+}
+
+extern "C" int bp_diff_set_log_params(bp_diff *x, const float *theta, void *stream)
+{
+    bp_clear_error();
+    if (!x || !theta) {
+        bp_set_error("bp_diff_set_log_params: NULL argument");
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipSetDevice(x->d->device));
+    if (x->num_rows)
+        hipLaunchKernelGGL(k_softmax, dim3(blocks_for(x->num_rows)), dim3(256), 0, st, theta, x->row_first_d,
+                           x->row_stride_d, x->P, x->num_rows, x->S);
+    size_t nT = (size_t)x->nrec * x->S;
+    if (nT)
+        hipLaunchKernelGGL(k_tables, dim3(blocks_for(nT)), dim3(256), 0, st, x->P, x->parents, x->par_def, x->def_tb,
+                           x->def_slice, x->d->J, x->nrec, x->S);
     size_t VS = (size_t)x->V * x->SP;
     if (VS)
         hipLaunchKernelGGL(k_initial, dim3(blocks_for(VS)), dim3(256), 0, st, x->P, x->prior_tb, x->d->states0,
@@ -944,6 +1009,21 @@ extern "C" int bp_diff_backward_ex(bp_diff *x, const float *grad_beliefs, float 
 extern "C" int bp_diff_backward(bp_diff *x, const float *grad_beliefs, float *grad_params, void *stream)
 {
     return bp_diff_backward_ex(x, grad_beliefs, grad_params, nullptr, nullptr, stream);
+}
+
+extern "C" int bp_diff_backward_log(bp_diff *x, const float *grad_beliefs, float *grad_theta, void *stream)
+{
+    if (!grad_theta) {
+        bp_set_error("bp_diff_backward_log: NULL argument");
+        return -1;
+    }
+    if (bp_diff_backward_ex(x, grad_beliefs, nullptr, nullptr, nullptr, stream))
+        return -1;
+    if (x->num_rows)
+        hipLaunchKernelGGL(k_softmax_back, dim3(blocks_for(x->num_rows)), dim3(256), 0, (hipStream_t)stream, x->P,
+                           x->gP, x->row_first_d, x->row_stride_d, grad_theta, x->num_rows, x->S);
+    HIP_TRY(hipGetLastError());
+    return 0;
 }
 
 extern "C" int bp_diff_tables(bp_diff *x, float *out)

@@ -454,6 +454,48 @@ __device__ __forceinline__ void load_row(const float *__restrict__ p, float *out
     }
 }
 
+/* Node phase for node v with in-slots [q0, q1) of the chunk's LDS staging
+ * (messages at lds[k * W + q]): the belief update of graph.c:985-1024
+ * (ITER, FWD), the combine buffer of graph.c:851-882 for the next
+ * iteration, or the edge schedule's product (EDGE). */
+template <int S, bool MIXED, int MODE>
+__device__ __forceinline__ void node_update(const LoopyArgs &a, unsigned v, unsigned q0, unsigned q1, float *st,
+                                            unsigned n, const float *lds, int W)
+{
+    float m[S], b[S];
+    if (MODE == ITER || MODE == FWD) {
+        float t[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            t[k] = 1.0f;
+        for (unsigned q = q0; q < q1; ++q) {
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+                m[k] = lds[k * W + q];
+            combine<S, MIXED>(t, m, n);
+        }
+        if (q1 > q0) {
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+                if (!MIXED || (unsigned)k < n)
+                    st[k] = st[k] * t[k];
+        }
+        normalise<S, MIXED>(st, n);
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        b[k] = st[k];
+    for (unsigned q = q0; q < q1; ++q) {
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            m[k] = lds[k * W + q];
+        combine_mode<S, MIXED, MODE>(b, m, n);
+    }
+    if (MODE == EDGE)
+        normalise_positive<S, MIXED>(b, n);
+    write_node<S, MIXED, MODE>(a, v, st, b, n);
+}
+
 /* A chunk of at most 64*R nodes with at most 64*R in-slots: lane l takes
  * slots sb+l+64r and nodes nb+l+64r (r < R).  Every load that does not
  * depend on another one is issued before any is consumed, so a wave keeps
@@ -542,42 +584,132 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         unsigned v = nb + r * WAVE + lane;
-        if (v < ne) {
-            unsigned n = nn[r];
-            float m[S], b[S];
-            if (MODE == ITER || MODE == FWD) {
-                float t[S];
+        if (v < ne)
+            node_update<S, MIXED, MODE>(a, v, q0[r], q1[r], st[r], nn[r], lds, W);
+    }
+}
+
+/* A chunk of at most 64 slots and 64 nodes with L lanes per slot: lane
+ * (g, k) = (lane / L, lane % L) forms rows i = k, k + L, ... of the message
+ * of slot g of each pass (L passes cover the chunk).  The normaliser is the
+ * sum of the rows in row order, gathered by shuffles, so every operation is
+ * the one send() performs, in its order.  Used for large state counts, where
+ * one lane per slot would read a whole table alone. */
+template <int S, bool MIXED, int MODE, int L, bool PACKED>
+__device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se,
+                                           float *lds, unsigned lane, double &dacc)
+{
+    constexpr int SP = Stride<S>::value;
+    constexpr int W = WAVE;
+    constexpr int G = WAVE / L;
+    constexpr int NR = (SP + L - 1) / L; /* rows per lane, pad rows included */
+    const unsigned k = lane % L, g = lane / L, base = lane - k;
+    unsigned v = nb + lane, q0 = 0, q1 = 0, n = S;
+    float st[S];
+    if (v < ne) {
+        q0 = a.in_off[v] - sb;
+        q1 = a.in_off[v + 1] - sb;
+        if (MODE == ITER || MODE == PREP || MODE == FWD)
+            load_row_nt<S>(a.states + (size_t)v * SP, st);
+        else
+            load_row<S>(a.states + (size_t)v * SP, st);
+        n = MIXED ? a.nvars[v] : S;
+    }
+#pragma unroll 1
+    for (int p = 0; p < L; ++p) {
+        unsigned q = p * G + g, s = sb + q;
+        bool act = s < se;
+        unsigned x = S, y = S;
+        float acc[NR];
 #pragma unroll
-                for (int k = 0; k < S; ++k)
-                    t[k] = 1.0f;
-                for (unsigned q = q0[r]; q < q1[r]; ++q) {
-#pragma unroll
-                    for (int k = 0; k < S; ++k)
-                        m[k] = lds[k * W + q];
-                    combine<S, MIXED>(t, m, n);
-                }
-                if (q1[r] > q0[r]) {
-#pragma unroll
-                    for (int k = 0; k < S; ++k)
-                        if (!MIXED || (unsigned)k < n)
-                            st[r][k] = st[r][k] * t[k];
-                }
-                normalise<S, MIXED>(st[r], n);
+        for (int r = 0; r < NR; ++r)
+            acc[r] = 0.0f;
+        if (act) {
+            if (MIXED) {
+                unsigned xy = a.slot_xy[s];
+                x = xy & 15u;
+                y = xy >> 4;
             }
+            if (MODE == PREP) {
 #pragma unroll
-            for (int k = 0; k < S; ++k)
-                b[k] = st[r][k];
-            for (unsigned q = q0[r]; q < q1[r]; ++q) {
+                for (int r = 0; r < NR; ++r) {
+                    unsigned i = k + r * L;
+                    if (i < (unsigned)S)
+                        acc[r] = a.msg_in[(size_t)s * SP + i];
+                }
+            } else {
+                unsigned u = a.slot_src[s];
+                const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
+                float b[S];
+                load_row<S>(src + (size_t)u * SP, b);
+                const float *T;
+                bool tr = false;
+                if constexpr (PACKED) {
+                    unsigned t = a.slot_tab[s];
+                    T = a.tabs + (t & 0x7fffffffu);
+                    tr = (t >> 31) != 0;
+                } else {
+                    T = a.J + (size_t)s * S * S;
+                }
+                const unsigned rstep = PACKED ? (tr ? 1u : y) : (unsigned)S, cstep = (PACKED && tr) ? x : 1u;
 #pragma unroll
-                for (int k = 0; k < S; ++k)
-                    m[k] = lds[k * W + q];
-                combine_mode<S, MIXED, MODE>(b, m, n);
+                for (int r = 0; r < NR; ++r) {
+                    unsigned i = k + r * L;
+                    float sacc = 0.0f;
+                    if (i < (unsigned)S && (!MIXED || i < x)) {
+                        const float *row = T + (size_t)i * rstep;
+#pragma unroll
+                        for (int j = 0; j < S; ++j)
+                            if (!MIXED || (unsigned)j < y)
+                                sacc = sacc + row[(size_t)j * cstep] * b[j];
+                    }
+                    acc[r] = sacc;
+                }
             }
-            if (MODE == EDGE)
-                normalise_positive<S, MIXED>(b, n);
-            write_node<S, MIXED, MODE>(a, v, st[r], b, n);
+        }
+        float m[NR];
+        if (MODE == PREP) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                m[r] = acc[r];
+        } else {
+            /* the row sums in row order, from the lanes that hold them */
+            float sum = 0.0f;
+#pragma unroll
+            for (int i = 0; i < S; ++i) {
+                float vi = __shfl(acc[i / L], (int)(base + (unsigned)(i % L)), WAVE);
+                if (!MIXED || (unsigned)i < x)
+                    sum = sum + vi;
+            }
+            if (sum <= 0.0f)
+                sum = 1.0f;
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                m[r] = acc[r] / sum;
+        }
+        if (act) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                unsigned i = k + r * L;
+                if (i >= (unsigned)SP)
+                    continue;
+                if (i < (unsigned)S) {
+                    if ((MODE == ITER || MODE == EDGE) && (!MIXED || i < x)) {
+                        float qd = a.msg_in[(size_t)s * SP + i] - m[r];
+                        if (qd != qd)
+                            qd = 0.0f;
+                        dacc += (double)fabsf(qd);
+                    }
+                    lds[i * W + q] = m[r];
+                }
+                if (MODE == ITER || MODE == INIT || MODE == EDGE)
+                    a.msg_out[(size_t)s * SP + i] = (i < (unsigned)S) ? m[r] : 0.0f;
+            }
         }
     }
+    wave_sync();
+    if (v < ne)
+        node_update<S, MIXED, MODE>(a, v, q0, q1, st, n, lds, W);
 }
 
 /* One node with more than 64 in-slots: its slots are walked in blocks of 64
@@ -654,12 +786,14 @@ template <int S> struct SlotsPerLane {
 
 /* One chunk: the narrow path, or the one-node path for a node with more
  * in-slots than a chunk holds. */
-template <int S, bool MIXED, int MODE, int R, bool PACKED>
+template <int S, bool MIXED, int MODE, int R, bool PACKED, int L>
 __device__ __forceinline__ void run_chunk(const LoopyArgs &a, unsigned c, float *lds, unsigned lane, double &dacc)
 {
     uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
     unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
-    if (se - sb <= (unsigned)(WAVE * R))
+    if (L > 1 && se - sb <= (unsigned)WAVE)
+        chunk_rows<S, MIXED, MODE, L, PACKED>(a, nb, ne, sb, se, lds, lane, dacc);
+    else if (L == 1 && se - sb <= (unsigned)(WAVE * R))
         chunk_narrow<S, MIXED, MODE, R, PACKED>(a, nb, ne, sb, se, lds, lane, dacc);
     else
         chunk_wide<S, MIXED, MODE, PACKED>(a, nb, sb, se, lds, lane, dacc);
@@ -676,7 +810,8 @@ __device__ __forceinline__ void flush_chunk_partial(const LoopyArgs &a, unsigned
     dacc = 0.0;
 }
 
-template <int S, bool MIXED, int MODE, int R = SlotsPerLane<S>::value, bool PACKED = false>
+/* L > 1: L lanes per slot over chunks of at most 64 slots and nodes (R = 1) */
+template <int S, bool MIXED, int MODE, int R = SlotsPerLane<S>::value, bool PACKED = false, int L = 1>
 __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
 {
     if (a.stop && *a.stop)
@@ -697,7 +832,7 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
                 break;
             unsigned end = base + a.qblock < a.nchunks ? base + a.qblock : a.nchunks;
             for (unsigned c = base; c < end; ++c) {
-                run_chunk<S, MIXED, MODE, R, PACKED>(a, c, lds[wave], lane, dacc);
+                run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], lane, dacc);
                 if (per_chunk)
                     flush_chunk_partial(a, c, lane, dacc);
             }
@@ -711,7 +846,7 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
         unsigned c0 = (unsigned)(((unsigned long long)w * a.nchunks) / W);
         unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / W);
         for (unsigned c = c0; c < c1; ++c) {
-            run_chunk<S, MIXED, MODE, R, PACKED>(a, c, lds[wave], lane, dacc);
+            run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], lane, dacc);
             if (per_chunk)
                 flush_chunk_partial(a, c, lane, dacc);
         }

@@ -74,7 +74,7 @@ def bench_adjoint(K: int = 30, c: int = 5, reps: int = 10) -> dict:
         th = theta.clone().requires_grad_(True)
         e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
         e[0].record()
-        b = net.beliefs(net.softmax_tables(th), K, c)
+        b = net.beliefs_from_log(th, K, c)
         loss = -torch.log(torch.clamp(b[q, t], min=1e-30)).mean()
         e[1].record()
         e[2].record()
@@ -140,7 +140,7 @@ def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: i
     out = {"workload": f"{count} networks x {evidence_sets} evidence sets: {info['V']} nodes, {info['E']} message "
                        f"edges, {K} iterations", "instances": int(info["instances"]),
            "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3, "speedup": t_plain / t_batch,
-           "shape": {"slots_per_lane": shape[0], "queue_block": shape[1]},
+           "shape": {"slots_per_lane": shape[0], "lanes_per_slot": shape[1], "queue_block": shape[2]},
            "iteration_kernel_ms": kern_ms,
            "iteration_achieved_GBs": info["alg_bytes"] / (kern_ms * 1e-3) / 1e9,
            "host_build_s": build_s,

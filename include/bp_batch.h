@@ -17,9 +17,10 @@
  * readers emit, and the same arcs under other evidence sets), points at it.
  *
  * The kernel's tile shape (slots per lane, i.e. edges per thread and chunk
- * size) and scheduling (static ranges, or a work queue handing out blocks
- * of chunks) are chosen by bp_batch_tune, which times the candidates on the
- * batch itself and can keep its choices in a JSON file.
+ * size; lanes per slot, i.e. threads per edge) and scheduling (static
+ * ranges, or a work queue handing out blocks of chunks) are chosen by
+ * bp_batch_tune, which times the candidates on the batch itself and can
+ * keep its choices in a JSON file.
  *
  * Every function returns 0, or -1 with the message in bp_last_error().
  */
@@ -48,11 +49,15 @@ int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index);
  * networks can be added afterwards. */
 int bp_batch_upload(bp_batch *b, int device);
 
-/* Kernel shape: slots_per_lane 1 or 2; queue_block 0 for static chunk
- * ranges, else the number of chunks a wave takes from the work queue at a
- * time. */
-int bp_batch_set_config(bp_batch *b, unsigned int slots_per_lane, unsigned int queue_block);
-int bp_batch_get_config(const bp_batch *b, unsigned int *slots_per_lane, unsigned int *queue_block);
+/* Kernel shape: slots_per_lane 1 or 2 (edges per thread; 2 needs six or
+ * fewer states); lanes_per_slot 1, 2, 4 or 8 (threads per edge, each
+ * forming every L-th row of the message; more than 1 needs four or more
+ * states and slots_per_lane 1); queue_block 0 for static chunk ranges, else
+ * the number of chunks a wave takes from the work queue at a time. */
+int bp_batch_set_config(bp_batch *b, unsigned int slots_per_lane, unsigned int lanes_per_slot,
+                        unsigned int queue_block);
+int bp_batch_get_config(const bp_batch *b, unsigned int *slots_per_lane, unsigned int *lanes_per_slot,
+                        unsigned int *queue_block);
 
 /* Times every candidate shape over `iterations` iterations and keeps the
  * fastest.  cache_path (nullable): a JSON file of earlier choices keyed by

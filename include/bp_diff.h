@@ -65,6 +65,11 @@ int bp_diff_rows(const bp_diff *d, uint64_t *row_first, uint32_t *row_stride, ui
  * info[4] floats). */
 int bp_diff_set_params(bp_diff *d, const float *params, void *stream);
 
+/* The same from log-parameters: each row of bp_diff_rows gets the softmax
+ * of its entries of theta (device, info[4] floats), and those are the
+ * parameters. */
+int bp_diff_set_log_params(bp_diff *d, const float *theta, void *stream);
+
 /* K iterations from the initial beliefs, checkpoint interval c >= 1;
  * beliefs (device, [V][stride]) receives the beliefs after iteration K. */
 int bp_diff_forward(bp_diff *d, unsigned int K, unsigned int c, float *beliefs, void *stream);
@@ -72,6 +77,10 @@ int bp_diff_forward(bp_diff *d, unsigned int K, unsigned int c, float *beliefs, 
 /* dL/dparams (device, info[4] floats, overwritten) from dL/dbeliefs
  * (device, [V][stride]) of the last bp_diff_forward. */
 int bp_diff_backward(bp_diff *d, const float *grad_beliefs, float *grad_params, void *stream);
+
+/* dL/dtheta (device, info[4] floats) after bp_diff_set_log_params: the
+ * adjoint, then the softmax's transpose per row. */
+int bp_diff_backward_log(bp_diff *d, const float *grad_beliefs, float *grad_theta, void *stream);
 
 /* Same as bp_diff_backward, also returning dL/dJ per message edge (device,
  * [E][S*S] in the reference's edge order, entry [i][j] with i the source

@@ -74,9 +74,10 @@ def test_every_shape_gives_the_same_bits(lib, preset_hi):
         gs.append(g)
     b.upload(0)
     ref = None
-    shapes = [(1, 0), (1, 1), (1, 4), (1, 16)] + ([(2, 0), (2, 4)] if preset_hi <= 6 else [])
-    for R, q in shapes:
-        b.set_config(R, q)
+    shapes = [(1, 1, 0), (1, 1, 1), (1, 1, 4), (1, 1, 16), (1, 2, 0), (1, 4, 4), (1, 8, 0)]
+    shapes += [(2, 1, 0), (2, 1, 4)] if preset_hi <= 6 else []
+    for R, L, q in shapes:
+        b.set_config(R, L, q)
         b.init()
         d = b.iterate(12, want_deltas=True)
         b.download(len(gs) - 1, gs[-1])
@@ -84,8 +85,8 @@ def test_every_shape_gives_the_same_bits(lib, preset_hi):
         if ref is None:
             ref = got
         else:
-            assert same_bits(got[0], ref[0]), f"shape {R},{q}"
-            assert np.array_equal(got[1], ref[1]), f"shape {R},{q} deltas"
+            assert same_bits(got[0], ref[0]), f"shape {R},{L},{q}"
+            assert np.array_equal(got[1], ref[1]), f"shape {R},{L},{q} deltas"
     b.free()
     for g in gs:
         lib.graph_destroy(g)
@@ -100,8 +101,8 @@ def test_tuner_picks_and_caches_a_shape(lib, tmp_path):
     data = json.loads(cache.read_text())
     assert len(data) == 1
     (entry,) = data.values()
-    assert (entry["slots_per_lane"], entry["queue_block"]) == first
-    b.set_config(1, 16 if first != (1, 16) else 0)
+    assert (entry["slots_per_lane"], entry["lanes_per_slot"], entry["queue_block"]) == first
+    b.set_config(1, 1, 16 if first != (1, 1, 16) else 0)
     b.tune(3, str(cache))  # a hit: no timing, same choice
     assert b.config() == first
     b.init()

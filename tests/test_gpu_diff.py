@@ -176,6 +176,32 @@ def test_autograd_through_softmax_parameters(lib):
     teardown(lib, bn, g, net)
 
 
+def test_fused_softmax_matches_torch_softmax(lib):
+    bn, A, g, G, net = setup("c5", num_nodes=2000)
+    gen = torch.Generator().manual_seed(8)
+    theta = torch.randn(net.num_params, generator=gen, dtype=torch.float32).cuda()
+    q, t = queries(G, net.S, 300, seed=8)
+    qd, td = torch.as_tensor(q, device="cuda"), torch.as_tensor(t, device="cuda")
+    K = 9
+    th1 = theta.clone().requires_grad_(True)
+    b1 = net.beliefs(net.softmax_tables(th1), K, 3)
+    ce(b1, qd, td).backward()
+    J1 = net.tables_host().astype(np.float64)
+    th2 = theta.clone().requires_grad_(True)
+    b2 = net.beliefs_from_log(th2, K, 3)
+    ce(b2, qd, td).backward()
+    J2 = net.tables_host().astype(np.float64)
+    # expf and the row sums may round differently from torch's softmax: the
+    # tables agree to f32 rounding, and the recursion carries that along
+    assert np.abs(J1 - J2).max() <= 1e-5 * np.abs(J1).max()
+    assert float((b1 - b2).abs().max()) < 1e-4
+    g1, g2 = th1.grad.double(), th2.grad.double()
+    rel = float((g1 - g2).norm() / g1.norm())
+    print(f"fused softmax: beliefs max |diff| {float((b1 - b2).abs().max()):.2e}, gradient rel l2 {rel:.2e}")
+    assert rel < 1e-3
+    teardown(lib, bn, g, net)
+
+
 def test_full_scale_c5(lib):
     """200k nodes, K = 30, checkpoints every 5: gradient against the float64
     oracle by directional finite differences; forward/backward times."""
@@ -188,7 +214,7 @@ def test_full_scale_c5(lib):
     K = 30
     th = theta.cuda().requires_grad_(True)
     qd, td = torch.as_tensor(q, device="cuda"), torch.as_tensor(t, device="cuda")
-    loss = ce(net.beliefs(net.softmax_tables(th), K, 5), qd, td)
+    loss = ce(net.beliefs_from_log(th, K, 5), qd, td)
     loss.backward()
     gr = th.grad.double().cpu().numpy()
     rows, inv = net._rows.cpu(), net._inv.cpu()
