@@ -163,6 +163,7 @@ _SIGNATURES = {
     "bp_last_error": (C.c_char_p, []),
     "bp_set_abort_on_error": (C.c_int, [C.c_int]),
     "bp_device_count": (C.c_int, []),
+    "bp_set_node_order": (C.c_int, [C.c_int]),
     "loopy_propagate_until_hip": (C.c_uint, [GraphP, C.c_float, C.c_uint]),
     "loopy_propagate_until_hip_edge": (C.c_uint, [GraphP, C.c_float, C.c_uint]),
     "bp_upload": (C.c_int, [GraphP, C.c_int, C.POINTER(C.c_void_p)]),

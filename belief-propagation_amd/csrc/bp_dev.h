@@ -52,9 +52,11 @@ struct bp_dev {
     unsigned *stop = nullptr;
 
     /* host mirrors for upload / download */
+    std::vector<unsigned> perm;      /* node -> device node (the device numbers nodes in its own order) */
+    std::vector<unsigned> inv;       /* device node -> node */
     std::vector<unsigned> slot_edge; /* slot -> reference edge index */
     std::vector<unsigned> edge_slot; /* reference edge index -> slot */
-    std::vector<unsigned> node_n;    /* state count per node */
+    std::vector<unsigned> node_n;    /* state count per device node */
     std::vector<unsigned> edge_x;    /* x per reference edge */
     double alg_bytes = 0.0;
 

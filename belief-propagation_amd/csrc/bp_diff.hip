@@ -77,34 +77,45 @@ __global__ void k_copy(const float *__restrict__ src, float *__restrict__ dst, s
         dst[i] = src[i];
 }
 
-/* One thread per (parent record, child state k): J_fwd[j][k] for every
- * parent state j, each the sequential f32 sum over the parent's blocks in
- * ascending TABLE order -- the reader's loop -- written to the forward slot
- * and, transposed, to the reverse slot. */
+/* One thread per (parent record, child state k, parent state j):
+ * J_fwd[j][k] = the sequential f32 sum of TABLE[k * slice + idx] over the
+ * configurations idx in which the parent is in state j, in ascending idx
+ * order -- the blocks of the reader's loop, in its order -- written to the
+ * forward slot and, transposed, to the reverse slot.  Loads are issued eight
+ * at a time ahead of the adds, which stay in order. */
 __global__ void k_tables(const float *__restrict__ P, const ParentRec *__restrict__ parents,
                          const unsigned *__restrict__ par_def, const unsigned *__restrict__ def_tb,
                          const unsigned *__restrict__ def_slice, float *__restrict__ J, unsigned nrec, unsigned S)
 {
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nrec * S)
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nrec * S * S)
         return;
-    unsigned pr = t / S, k = t % S;
+    unsigned pr = (unsigned)(t / ((size_t)S * S)), kj = (unsigned)(t % ((size_t)S * S));
+    unsigned k = kj / S, j = kj % S;
     ParentRec rec = parents[pr];
+    if (j >= rec.n_par)
+        return;
     unsigned d = par_def[pr];
-    unsigned slice = def_slice[d], offset = rec.offset, npar = rec.n_par, stride = npar * offset;
-    const float *T = P + def_tb[d] + (size_t)k * slice;
-    size_t SS = (size_t)S * S;
-    for (unsigned j = 0; j < npar; ++j) {
-        float acc = 0.0f;
-        for (unsigned b = 0; j * offset + b * stride < slice; ++b) {
-            const float *blk = T + j * offset + b * stride;
-            for (unsigned q = 0; q < offset; ++q)
-                acc += blk[q];
+    unsigned slice = def_slice[d], o = rec.offset, n = rec.n_par;
+    const float *T = P + def_tb[d] + (size_t)k * slice + (size_t)j * o;
+    unsigned cnt = slice / n; /* entries with the parent in state j */
+    float acc = 0.0f;
+    for (unsigned m0 = 0; m0 < cnt; m0 += 8) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            unsigned m = m0 + u;
+            v[u] = (m < cnt) ? T[(size_t)(m / o) * n * o + (m % o)] : 0.0f;
         }
-        J[rec.slot_f * SS + j * S + k] = acc;
-        if (rec.slot_r != NONE)
-            J[rec.slot_r * SS + k * S + j] = acc;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (m0 + u < cnt)
+                acc += v[u];
     }
+    size_t SS = (size_t)S * S;
+    J[rec.slot_f * SS + j * S + k] = acc;
+    if (rec.slot_r != NONE)
+        J[rec.slot_r * SS + k * S + j] = acc;
 }
 
 __global__ void k_initial(const float *__restrict__ P, const unsigned *__restrict__ prior_tb,
@@ -147,7 +158,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
             if (s < se) {
                 unsigned u = a.slot_src[s];
                 load_row<S * S>(a.J + (size_t)s * S * S, Jr[r]);
-                load_row<S>(a.bsrc + (size_t)u * SP, b[r]);
+                load_rowp<S, SP>(a.bsrc + (size_t)u * SP, b[r]);
             }
         }
 #pragma unroll
@@ -184,8 +195,8 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
                 unsigned q0 = a.in_off[v] - sb, q1 = a.in_off[v + 1] - sb;
                 unsigned o0 = a.out_off[v], o1 = a.out_off[v + 1];
                 float st[S], g[S], T[S], gb[S], dT[S];
-                load_row<S>(a.st_cur + (size_t)v * SP, st);
-                load_row<S>(a.gst_in + (size_t)v * SP, g);
+                load_rowp<S, SP>(a.st_cur + (size_t)v * SP, st);
+                load_rowp<S, SP>(a.gst_in + (size_t)v * SP, g);
 #pragma unroll
                 for (int k = 0; k < S; ++k) {
                     T[k] = 1.0f;
@@ -202,7 +213,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
                 if (a.w_in) {
                     for (unsigned o = o0; o < o1; ++o) {
                         float x[S];
-                        load_row<S>(a.w_in + (size_t)a.out_slot[o] * SP, x);
+                        load_rowp<S, SP>(a.w_in + (size_t)a.out_slot[o] * SP, x);
 #pragma unroll
                         for (int k = 0; k < S; ++k)
                             gb[k] = gb[k] + x[k];
@@ -217,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
                     }
                 } else {
                     float sp[S], ds[S], av[S];
-                    load_row<S>(a.st_prev + (size_t)v * SP, sp);
+                    load_rowp<S, SP>(a.st_prev + (size_t)v * SP, sp);
                     bool indeg = q1 > q0;
                     float sa = 0.0f, dot = 0.0f;
 #pragma unroll
@@ -329,7 +340,8 @@ __global__ void k_initial_grad(const float *__restrict__ part, const float *__re
     }
 }
 
-/* dL/dparam: the transpose of k_tables and k_initial */
+/* dL/dparam: the transpose of k_tables and k_initial.  The parent records
+ * and dJ entries of up to eight parents are loaded before any is summed. */
 __global__ void k_param_grad(const float *__restrict__ jbar, const float *__restrict__ dst0,
                              const unsigned *__restrict__ def_of, const unsigned *__restrict__ def_tb,
                              const unsigned *__restrict__ def_slice, const unsigned *__restrict__ def_pbegin,
@@ -342,13 +354,27 @@ __global__ void k_param_grad(const float *__restrict__ jbar, const float *__rest
     unsigned d = def_of[p];
     unsigned local = p - def_tb[d], slice = def_slice[d];
     unsigned k = local / slice, idx = local % slice;
+    unsigned i0 = def_pbegin[d], i1 = def_pbegin[d + 1];
+    size_t SS = (size_t)S * S;
     float g = 0.0f;
-    for (unsigned i = def_pbegin[d]; i < def_pbegin[d + 1]; ++i) {
-        ParentRec pr = parents[i];
-        unsigned j = (idx / pr.offset) % pr.n_par;
-        g = g + jbar[(size_t)pr.slot_f * S * S + j * S + k];
-        if (pr.slot_r != NONE)
-            g = g + jbar[(size_t)pr.slot_r * S * S + k * S + j];
+    for (unsigned base = i0; base < i1; base += 8) {
+        float f[8], r[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            f[u] = 0.0f;
+            r[u] = 0.0f;
+            if (base + u < i1) {
+                ParentRec pr = parents[base + u];
+                unsigned j = (idx / pr.offset) % pr.n_par;
+                f[u] = jbar[(size_t)pr.slot_f * SS + j * S + k];
+                if (pr.slot_r != NONE)
+                    r[u] = jbar[(size_t)pr.slot_r * SS + k * S + j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (base + u < i1)
+                g = g + f[u] + r[u];
     }
     unsigned v = def_prior[d];
     if (v != NONE)
@@ -393,6 +419,18 @@ __global__ void k_softmax_back(const float *__restrict__ P, const float *__restr
         gtheta[f + k * st] = P[f + k * st] * (gP[f + k * st] - dot);
 }
 
+/* dst[i] = src[idx[i]] for rows of SP floats (between the caller's node
+ * numbering and the device's) */
+__global__ void k_gather_rows(const float *__restrict__ src, const unsigned *__restrict__ idx,
+                              float *__restrict__ dst, unsigned V, unsigned SP)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)V * SP)
+        return;
+    unsigned i = (unsigned)(t / SP), k = (unsigned)(t % SP);
+    dst[t] = src[(size_t)idx[i] * SP + k];
+}
+
 /* slot order -> reference edge order */
 __global__ void k_slot_to_edge(const float *__restrict__ src, const unsigned *__restrict__ slot_edge,
                                float *__restrict__ dst, unsigned E, unsigned width)
@@ -423,6 +461,7 @@ struct bp_diff {
     ParentRec *parents = nullptr;
 
     float *P = nullptr;     /* parameters */
+    unsigned *perm_d = nullptr, *inv_d = nullptr; /* node -> device node, device node -> node */
     unsigned *row_first_d = nullptr, *row_stride_d = nullptr; /* softmax rows */
     float *st0 = nullptr;   /* initial beliefs */
     float *gP = nullptr;    /* dL/dparams */
@@ -470,7 +509,7 @@ extern "C" void bp_diff_free(bp_diff *x)
         (void)hipDeviceSynchronize();
     }
     diff_release_storage(x);
-    void *ps[] = {x->row_first_d, x->row_stride_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
+    void *ps[] = {x->perm_d, x->inv_d, x->row_first_d, x->row_stride_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
                   x->def_pbegin, x->def_prior, x->parents, x->P, x->st0, x->gP, x->gin, x->gst[0], x->gst[1],
                   x->w[0], x->w[1], x->jbar, x->dst0p, x->dst0};
     for (void *p : ps)
@@ -594,10 +633,11 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
     for (unsigned dd = 0; dd < bn->num_defs; ++dd)
         if (bn->def_given_begin[dd + 1] == bn->def_given_begin[dd])
             state_def[bn->def_for[dd]] = dd;
+    /* per device node (the handle numbers nodes in its own order) */
     for (unsigned v = 0; v < V; ++v)
         if (state_def[v] != NONE && state_def[v] < npd) {
-            prior_tb[v] = def_tb[state_def[v]];
-            def_prior[state_def[v]] = v;
+            prior_tb[d->perm[v]] = def_tb[state_def[v]];
+            def_prior[state_def[v]] = d->perm[v];
         }
     /* softmax rows */
     x->row_first.clear();
@@ -621,7 +661,7 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
     /* source-grouped slots */
     std::vector<unsigned> out_off(V + 1, 0), out_slot(E), slot_src(E);
     for (unsigned s = 0; s < E; ++s) {
-        slot_src[s] = g->edges_src_index[d->slot_edge[s]];
+        slot_src[s] = d->perm[g->edges_src_index[d->slot_edge[s]]];
         out_off[slot_src[s] + 1]++;
     }
     for (unsigned v = 0; v < V; ++v)
@@ -632,7 +672,8 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
             out_slot[fill[slot_src[s]]++] = s;
     }
     size_t VS = (size_t)V * d->SP, ES = (size_t)E * d->SP;
-    if (upload_vec(&x->prior_tb, prior_tb) || upload_vec(&x->out_off, out_off) ||
+    if (upload_vec(&x->perm_d, d->perm) || upload_vec(&x->inv_d, d->inv) || upload_vec(&x->prior_tb, prior_tb) ||
+        upload_vec(&x->out_off, out_off) ||
         upload_vec(&x->out_slot, out_slot) || upload_vec(&x->slot_edge, d->slot_edge) ||
         upload_vec(&x->def_of, def_of) || upload_vec(&x->def_tb, def_tb) || upload_vec(&x->def_slice, def_slice) ||
         upload_vec(&x->def_pbegin, def_pbegin) || upload_vec(&x->def_prior, def_prior) ||
@@ -729,7 +770,7 @@ extern "C" int bp_diff_set_params(bp_diff *x, const float *params, void *stream)
     if (x->num_params)
         hipLaunchKernelGGL(k_copy, dim3(blocks_for(x->num_params)), dim3(256), 0, st, params, x->P,
                            (size_t)x->num_params);
-    size_t nT = (size_t)x->nrec * x->S;
+    size_t nT = (size_t)x->nrec * x->S * x->S;
     if (nT)
         hipLaunchKernelGGL(k_tables, dim3(blocks_for(nT)), dim3(256), 0, st, x->P, x->parents, x->par_def, x->def_tb,
                            x->def_slice, x->d->J, x->nrec, x->S);
@@ -755,7 +796,7 @@ extern "C" int bp_diff_set_log_params(bp_diff *x, const float *theta, void *stre
     if (x->num_rows)
         hipLaunchKernelGGL(k_softmax, dim3(blocks_for(x->num_rows)), dim3(256), 0, st, theta, x->row_first_d,
                            x->row_stride_d, x->P, x->num_rows, x->S);
-    size_t nT = (size_t)x->nrec * x->S;
+    size_t nT = (size_t)x->nrec * x->S * x->S;
     if (nT)
         hipLaunchKernelGGL(k_tables, dim3(blocks_for(nT)), dim3(256), 0, st, x->P, x->parents, x->par_def, x->def_tb,
                            x->def_slice, x->d->J, x->nrec, x->S);
@@ -875,7 +916,8 @@ extern "C" int bp_diff_forward(bp_diff *x, unsigned int K, unsigned int c, float
         it = x->fwd_graphs.emplace(key, exec).first;
     }
     HIP_TRY(hipGraphLaunch(it->second, st));
-    hipLaunchKernelGGL(k_copy, dim3(blocks_for(VS)), dim3(256), 0, st, x->final_st, beliefs, VS);
+    hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(VS)), dim3(256), 0, st, x->final_st, x->perm_d, beliefs, x->V,
+                       x->SP);
     HIP_TRY(hipGetLastError());
     x->have_forward = true;
     return 0;
@@ -980,7 +1022,8 @@ extern "C" int bp_diff_backward_ex(bp_diff *x, const float *grad_beliefs, float 
     HIP_TRY(hipSetDevice(x->d->device));
     hipStream_t st = (hipStream_t)stream;
     size_t VS = (size_t)x->V * x->SP;
-    hipLaunchKernelGGL(k_copy, dim3(blocks_for(VS)), dim3(256), 0, st, grad_beliefs, x->gin, VS);
+    hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(VS)), dim3(256), 0, st, grad_beliefs, x->inv_d, x->gin, x->V,
+                       x->SP);
     auto key = std::make_pair(x->K, x->c);
     auto it = x->bwd_graphs.find(key);
     if (it == x->bwd_graphs.end()) {
@@ -1001,7 +1044,8 @@ extern "C" int bp_diff_backward_ex(bp_diff *x, const float *grad_beliefs, float 
         hipLaunchKernelGGL(k_slot_to_edge, dim3(blocks_for((size_t)x->E * SS)), dim3(256), 0, st, x->jbar,
                            x->slot_edge, grad_tables, x->E, (unsigned)SS);
     if (grad_initial)
-        hipLaunchKernelGGL(k_copy, dim3(blocks_for(VS)), dim3(256), 0, st, x->dst0, grad_initial, VS);
+        hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for(VS)), dim3(256), 0, st, x->dst0, x->perm_d, grad_initial,
+                           x->V, x->SP);
     HIP_TRY(hipGetLastError());
     return 0;
 }

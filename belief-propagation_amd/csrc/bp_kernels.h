@@ -496,6 +496,31 @@ __device__ __forceinline__ void node_update(const LoopyArgs &a, unsigned v, unsi
     write_node<S, MIXED, MODE>(a, v, st, b, n);
 }
 
+/* The first S floats of a row of SP: one 16-byte load when a 3-state row
+ * sits in a 4-float slot, else load_row. */
+template <int S, int SP>
+__device__ __forceinline__ void load_rowp(const float *__restrict__ p, float *out)
+{
+    if constexpr (S == 3 && SP == 4) {
+        float4 v = *reinterpret_cast<const float4 *>(p);
+        out[0] = v.x;
+        out[1] = v.y;
+        out[2] = v.z;
+    } else {
+        load_row<S>(p, out);
+    }
+}
+
+/* a streamed row (non-temporal when BP_NT_LOADS is set) */
+template <int S, int SP>
+__device__ __forceinline__ void load_rowp_nt(const float *__restrict__ p, float *out)
+{
+    if constexpr (BP_NT_LOADS)
+        load_row_nt<S>(p, out);
+    else
+        load_rowp<S, SP>(p, out);
+}
+
 /* A chunk of at most 64*R nodes with at most 64*R in-slots: lane l takes
  * slots sb+l+64r and nodes nb+l+64r (r < R).  Every load that does not
  * depend on another one is issued before any is consumed, so a wave keeps
@@ -522,7 +547,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                     load_table<S, PACKED>(a, s, xy[r], J[r]);
             }
             if (MODE == ITER || MODE == EDGE || MODE == PREP)
-                load_row_nt<S>(a.msg_in + (size_t)s * SP, old[r]);
+                load_rowp_nt<S, SP>(a.msg_in + (size_t)s * SP, old[r]);
         }
         unsigned v = nb + r * WAVE + lane;
         if (v < ne) {
@@ -530,9 +555,9 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
             q1[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v + 1) : a.in_off[v + 1]) - sb;
             /* INIT and EDGE gather beliefs: keep them cached there */
             if (MODE == ITER || MODE == PREP || MODE == FWD)
-                load_row_nt<S>(a.states + (size_t)v * SP, st[r]);
+                load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st[r]);
             else
-                load_row<S>(a.states + (size_t)v * SP, st[r]);
+                load_rowp<S, SP>(a.states + (size_t)v * SP, st[r]);
             nn[r] = MIXED ? a.nvars[v] : S;
         }
     }
@@ -549,7 +574,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
             } else {
                 const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
                 float b[S];
-                load_row<S>(src + (size_t)u[r] * SP, b);
+                load_rowp<S, SP>(src + (size_t)u[r] * SP, b);
                 unsigned x = xy[r] & 15u, y = xy[r] >> 4;
                 if constexpr (HOLD)
                     send<S, MIXED>(J[r], b, x, y, m);

@@ -224,6 +224,8 @@ struct PlanBuilder {
 
 int upload_plan(bp_dev *d, PlanBuilder &pb, bp_tree_plan **out)
 {
+    for (unsigned &v : pb.entries) /* the kernels index device nodes */
+        v = d->perm[v];
     bp_tree_plan *p = new bp_tree_plan();
     p->wave_begin = pb.wave_begin;
     p->wave_start = pb.wave_start;

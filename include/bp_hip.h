@@ -49,6 +49,14 @@ int bp_set_abort_on_error(int enabled); /* returns the previous setting */
 /* Number of HIP devices visible (0 without a GPU); never fails. */
 int bp_device_count(void);
 
+/* Node numbering on the device for handles uploaded from now on: 1 (the
+ * default) renumbers nodes in reverse Cuthill-McKee order for locality of
+ * the gathers, 0 keeps the graph's order.  Results are the same either way
+ * (each node's in-edges keep the reference's order); only the FP64 delta's
+ * summation order, and so its last bits, follow the numbering.  Returns the
+ * previous setting. */
+int bp_set_node_order(int mode);
+
 /* GPU propagation with the contract of the reference's CUDA entry point:
  * the caller has run init_previous_edge (as src/cuda_benchmark/
  * belief-propagation.cu:1081 does); g is uploaded, iterated until the stop

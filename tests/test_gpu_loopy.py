@@ -308,3 +308,25 @@ def test_convergence_drivers(lib, orc, variant):
     assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
     lib.graph_destroy(g)
     lib.graph_destroy(h)
+
+
+@pytest.mark.parametrize("case", ["c1", "lattice16", "bf_1000_2000_1"])
+def test_device_node_order_does_not_change_results(lib, case):
+    """Handles renumber nodes on the device (reverse Cuthill-McKee by
+    default); the reference's order gives the same bits, and deltas that
+    differ only in the FP64 summation order."""
+    d = golden(case)
+    out = []
+    for mode in (0, 1):
+        prev = lib.bp_set_node_order(mode)
+        g = graph_for_case(d)
+        dev = B.Device(g)
+        lib.bp_set_node_order(prev)
+        dev.init()
+        deltas = dev.iterate(15, want_deltas=True)
+        dev.download(g)
+        out.append((B.states(g).copy(), B.messages(g, "prev").copy(), deltas))
+        dev.free()
+        lib.graph_destroy(g)
+    assert same_bits(out[0][0], out[1][0]) and same_bits(out[0][1], out[1][1])
+    assert np.allclose(out[0][2], out[1][2], rtol=1e-12, atol=0)
