@@ -622,7 +622,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
  * one lane per slot would read a whole table alone. */
 template <int S, bool MIXED, int MODE, int L, bool PACKED>
 __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se,
-                                           float *lds, unsigned lane, double &dacc)
+                                           float *lds, float *stage_f, unsigned *stage_u, unsigned lane, double &dacc)
 {
     constexpr int SP = Stride<S>::value;
     constexpr int W = WAVE;
@@ -631,15 +631,44 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
     const unsigned k = lane % L, g = lane / L, base = lane - k;
     unsigned v = nb + lane, q0 = 0, q1 = 0, n = S;
     float st[S];
+    /* stage every slot's operands once, one lane per slot (coalesced, and
+     * the gathers issued together): the source's row b, the old message,
+     * the table reference and the state counts */
+    {
+        unsigned s = sb + lane;
+        if (s < se) {
+            unsigned xy = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
+            unsigned t = PACKED ? a.slot_tab[s] : 0u;
+            float b[S], old[S];
+            if (MODE == PREP) {
+                load_rowp<S, SP>(a.msg_in + (size_t)s * SP, b);
+            } else {
+                unsigned u = a.slot_src[s];
+                const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
+                load_rowp<S, SP>(src + (size_t)u * SP, b);
+            }
+            if (MODE == ITER || MODE == EDGE)
+                load_rowp<S, SP>(a.msg_in + (size_t)s * SP, old);
+#pragma unroll
+            for (int j = 0; j < S; ++j) {
+                stage_f[j * W + lane] = b[j];
+                if (MODE == ITER || MODE == EDGE)
+                    stage_f[(S + j) * W + lane] = old[j];
+            }
+            stage_u[lane] = xy;
+            stage_u[W + lane] = t;
+        }
+    }
     if (v < ne) {
         q0 = a.in_off[v] - sb;
         q1 = a.in_off[v + 1] - sb;
         if (MODE == ITER || MODE == PREP || MODE == FWD)
-            load_row_nt<S>(a.states + (size_t)v * SP, st);
+            load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st);
         else
-            load_row<S>(a.states + (size_t)v * SP, st);
+            load_rowp<S, SP>(a.states + (size_t)v * SP, st);
         n = MIXED ? a.nvars[v] : S;
     }
+    wave_sync();
 #pragma unroll 1
     for (int p = 0; p < L; ++p) {
         unsigned q = p * G + g, s = sb + q;
@@ -651,7 +680,7 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
             acc[r] = 0.0f;
         if (act) {
             if (MIXED) {
-                unsigned xy = a.slot_xy[s];
+                unsigned xy = stage_u[q];
                 x = xy & 15u;
                 y = xy >> 4;
             }
@@ -660,17 +689,17 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                 for (int r = 0; r < NR; ++r) {
                     unsigned i = k + r * L;
                     if (i < (unsigned)S)
-                        acc[r] = a.msg_in[(size_t)s * SP + i];
+                        acc[r] = stage_f[i * W + q];
                 }
             } else {
-                unsigned u = a.slot_src[s];
-                const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
                 float b[S];
-                load_row<S>(src + (size_t)u * SP, b);
+#pragma unroll
+                for (int j = 0; j < S; ++j)
+                    b[j] = stage_f[j * W + q];
                 const float *T;
                 bool tr = false;
                 if constexpr (PACKED) {
-                    unsigned t = a.slot_tab[s];
+                    unsigned t = stage_u[W + q];
                     T = a.tabs + (t & 0x7fffffffu);
                     tr = (t >> 31) != 0;
                 } else {
@@ -720,7 +749,7 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                     continue;
                 if (i < (unsigned)S) {
                     if ((MODE == ITER || MODE == EDGE) && (!MIXED || i < x)) {
-                        float qd = a.msg_in[(size_t)s * SP + i] - m[r];
+                        float qd = stage_f[(S + i) * W + q] - m[r];
                         if (qd != qd)
                             qd = 0.0f;
                         dacc += (double)fabsf(qd);
@@ -812,12 +841,13 @@ template <int S> struct SlotsPerLane {
 /* One chunk: the narrow path, or the one-node path for a node with more
  * in-slots than a chunk holds. */
 template <int S, bool MIXED, int MODE, int R, bool PACKED, int L>
-__device__ __forceinline__ void run_chunk(const LoopyArgs &a, unsigned c, float *lds, unsigned lane, double &dacc)
+__device__ __forceinline__ void run_chunk(const LoopyArgs &a, unsigned c, float *lds, float *stage_f,
+                                          unsigned *stage_u, unsigned lane, double &dacc)
 {
     uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
     unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
     if (L > 1 && se - sb <= (unsigned)WAVE)
-        chunk_rows<S, MIXED, MODE, L, PACKED>(a, nb, ne, sb, se, lds, lane, dacc);
+        chunk_rows<S, MIXED, MODE, L, PACKED>(a, nb, ne, sb, se, lds, stage_f, stage_u, lane, dacc);
     else if (L == 1 && se - sb <= (unsigned)(WAVE * R))
         chunk_narrow<S, MIXED, MODE, R, PACKED>(a, nb, ne, sb, se, lds, lane, dacc);
     else
@@ -842,6 +872,9 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
     if (a.stop && *a.stop)
         return;
     __shared__ float lds[WAVES_PER_BLOCK][S * WAVE * R];
+    /* lanes-per-slot path: staged slot operands (rows b and old, state counts, tables) */
+    __shared__ float stage_f[WAVES_PER_BLOCK][L > 1 ? 2 * S * WAVE : 1];
+    __shared__ unsigned stage_u[WAVES_PER_BLOCK][L > 1 ? 2 * WAVE : 1];
     __shared__ double red[WAVES_PER_BLOCK];
     unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     double dacc = 0.0;
@@ -857,7 +890,7 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
                 break;
             unsigned end = base + a.qblock < a.nchunks ? base + a.qblock : a.nchunks;
             for (unsigned c = base; c < end; ++c) {
-                run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], lane, dacc);
+                run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], stage_f[wave], stage_u[wave], lane, dacc);
                 if (per_chunk)
                     flush_chunk_partial(a, c, lane, dacc);
             }
@@ -871,7 +904,7 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
         unsigned c0 = (unsigned)(((unsigned long long)w * a.nchunks) / W);
         unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / W);
         for (unsigned c = c0; c < c1; ++c) {
-            run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], lane, dacc);
+            run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], stage_f[wave], stage_u[wave], lane, dacc);
             if (per_chunk)
                 flush_chunk_partial(a, c, lane, dacc);
         }
