@@ -44,9 +44,10 @@ struct bp_dev {
     float *states0 = nullptr;         /* [V][SP] the states given at upload, for bp_restore_states */
     uint2 *chunk = nullptr;     /* [nchunks+1] (first node, first slot) */
     unsigned nchunks = 0, G = 0;
+    unsigned NP = 0;            /* delta partials per iteration (one per wave) */
     unsigned part_cap = 1;      /* iterations whose delta partials fit the buffer */
 
-    double *partials = nullptr; /* [part_cap][G] */
+    double *partials = nullptr; /* [part_cap][NP] */
     double *deltas = nullptr;   /* [max(part_cap,16)] */
     double *last = nullptr;
     unsigned *stop = nullptr;

@@ -152,14 +152,23 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
         /* slot phase 1: the message of every slot, kept with its operands */
         float Jr[R][S * S], b[R][S], m[R][S], sig[R];
         bool pos[R];
+        unsigned u[R];
+        /* lanes past the chunk's end load a valid neighbour (results
+         * dropped): branch-free loads, all in flight before the gathers */
+        const bool any_slots = se > sb; /* uniform: no stand-in slot otherwise */
+        const unsigned s_last = se - 1;
+        if (any_slots) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            unsigned s = sb + r * WAVE + lane;
-            if (s < se) {
-                unsigned u = a.slot_src[s];
+            for (int r = 0; r < R; ++r) {
+                unsigned s = sb + r * WAVE + lane;
+                if (s >= se)
+                    s = s_last;
+                u[r] = a.slot_src[s];
                 load_row<S * S>(a.J + (size_t)s * S * S, Jr[r]);
-                load_rowp<S, SP>(a.bsrc + (size_t)u * SP, b[r]);
             }
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                load_rowp<S, SP>(a.bsrc + (size_t)u[r] * SP, b[r]);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {

@@ -90,7 +90,7 @@ struct LoopyArgs {
     float *msg_out;              /* [E][SP] new messages (ITER, INIT) */
     const uint2 *chunk;          /* [nchunks+1] (first node, first slot) of each chunk */
     unsigned nchunks;
-    double *partials;            /* [G] this launch's per-block delta partials (ITER) */
+    double *partials;            /* [G * waves per block] this launch's per-wave delta partials (ITER) */
     const unsigned *stop;        /* non-null: skip the launch once *stop != 0 */
     /* packed tables (batched networks): slot s's table starts at
      * tabs + (slot_tab[s] & 0x7fffffff), row-major with the slot's x rows and
@@ -536,10 +536,19 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
     constexpr bool HOLD = S * S * R <= 32;
     unsigned u[R], xy[R], q0[R], q1[R], nn[R];
     float J[HOLD ? R : 1][HOLD ? S * S : 1], old[R][S], st[R][S];
+    /* Lanes past the chunk's end read a valid neighbour instead (and drop
+     * the result), so the loads carry no branches and all of them -- then
+     * all the gathers -- are in flight before the first wait. */
+    /* (a chunk of nodes without in-edges has no slot to stand in: its slot
+     * loads are skipped, uniformly for the wave) */
+    const bool any_slots = se > sb;
+    const unsigned s_last = se - 1, v_last = ne - 1;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         unsigned s = sb + r * WAVE + lane;
-        if (s < se) {
+        if (s >= se)
+            s = s_last;
+        if (any_slots) {
             if (MODE != PREP) {
                 u[r] = BP_NT_LOADS ? __builtin_nontemporal_load(a.slot_src + s) : a.slot_src[s];
                 xy[r] = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
@@ -550,16 +559,23 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 load_rowp_nt<S, SP>(a.msg_in + (size_t)s * SP, old[r]);
         }
         unsigned v = nb + r * WAVE + lane;
-        if (v < ne) {
-            q0[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v) : a.in_off[v]) - sb;
-            q1[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v + 1) : a.in_off[v + 1]) - sb;
-            /* INIT and EDGE gather beliefs: keep them cached there */
-            if (MODE == ITER || MODE == PREP || MODE == FWD)
-                load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st[r]);
-            else
-                load_rowp<S, SP>(a.states + (size_t)v * SP, st[r]);
-            nn[r] = MIXED ? a.nvars[v] : S;
-        }
+        if (v >= ne)
+            v = v_last;
+        q0[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v) : a.in_off[v]) - sb;
+        q1[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v + 1) : a.in_off[v + 1]) - sb;
+        /* INIT and EDGE gather beliefs: keep them cached there */
+        if (MODE == ITER || MODE == PREP || MODE == FWD)
+            load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st[r]);
+        else
+            load_rowp<S, SP>(a.states + (size_t)v * SP, st[r]);
+        nn[r] = MIXED ? a.nvars[v] : S;
+    }
+    float bsrc[R][S];
+    if (MODE != PREP && any_slots) {
+        const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            load_rowp<S, SP>(src + (size_t)u[r] * SP, bsrc[r]);
     }
     /* slot phase: gather, send, write, delta, stage in LDS */
 #pragma unroll
@@ -572,14 +588,11 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 for (int k = 0; k < S; ++k)
                     m[k] = old[r][k];
             } else {
-                const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
-                float b[S];
-                load_rowp<S, SP>(src + (size_t)u[r] * SP, b);
                 unsigned x = xy[r] & 15u, y = xy[r] >> 4;
                 if constexpr (HOLD)
-                    send<S, MIXED>(J[r], b, x, y, m);
+                    send<S, MIXED>(J[r], bsrc[r], x, y, m);
                 else
-                    send_from_memory<S, MIXED, PACKED>(a, s, xy[r], b, m);
+                    send_from_memory<S, MIXED, PACKED>(a, s, xy[r], bsrc[r], m);
                 if (MODE == ITER || MODE == EDGE) {
 #pragma unroll
                     for (int i = 0; i < S; ++i) {
@@ -875,7 +888,6 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
     /* lanes-per-slot path: staged slot operands (rows b and old, state counts, tables) */
     __shared__ float stage_f[WAVES_PER_BLOCK][L > 1 ? 2 * S * WAVE : 1];
     __shared__ unsigned stage_u[WAVES_PER_BLOCK][L > 1 ? 2 * WAVE : 1];
-    __shared__ double red[WAVES_PER_BLOCK];
     unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
     double dacc = 0.0;
     const bool per_chunk = (MODE == ITER || MODE == EDGE) && a.chunk_partials != nullptr;
@@ -909,17 +921,12 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
                 flush_chunk_partial(a, c, lane, dacc);
         }
     }
+    /* one FP64 partial per wave, no workgroup barrier: a wave that is done
+     * leaves at once instead of waiting for the slowest of its group */
     if ((MODE == ITER || MODE == EDGE) && !per_chunk) {
         double d = wave_sum(dacc);
         if (lane == 0)
-            red[wave] = d;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int w2 = 0; w2 < WAVES_PER_BLOCK; ++w2)
-                t += red[w2];
-            a.partials[blockIdx.x] = t;
-        }
+            a.partials[blockIdx.x * WAVES_PER_BLOCK + wave] = d;
     }
 }
 

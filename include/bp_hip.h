@@ -50,7 +50,7 @@ int bp_set_abort_on_error(int enabled); /* returns the previous setting */
 int bp_device_count(void);
 
 /* Node numbering on the device for handles uploaded from now on: 1 (the
- * default) renumbers nodes in reverse Cuthill-McKee order for locality of
+ * default) renumbers nodes in Cuthill-McKee order for locality of
  * the gathers, 0 keeps the graph's order.  Results are the same either way
  * (each node's in-edges keep the reference's order); only the FP64 delta's
  * summation order, and so its last bits, follow the numbering.  Returns the

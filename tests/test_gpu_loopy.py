@@ -312,7 +312,7 @@ def test_convergence_drivers(lib, orc, variant):
 
 @pytest.mark.parametrize("case", ["c1", "lattice16", "bf_1000_2000_1"])
 def test_device_node_order_does_not_change_results(lib, case):
-    """Handles renumber nodes on the device (reverse Cuthill-McKee by
+    """Handles renumber nodes on the device (Cuthill-McKee by
     default); the reference's order gives the same bits, and deltas that
     differ only in the FP64 summation order."""
     d = golden(case)
