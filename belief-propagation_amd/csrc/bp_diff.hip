@@ -129,6 +129,30 @@ __global__ void k_initial(const float *__restrict__ P, const unsigned *__restric
     st0[t] = (tb != NONE && k < S) ? P[tb + k] : base[t];
 }
 
+/* acc += rows[out_slot[o]] for o in [o0, o1), in order; four index loads,
+ * then four row loads, are issued before any is added */
+template <int S, int SP>
+__device__ __forceinline__ void sum_out_rows(const float *__restrict__ rows, const unsigned *__restrict__ out_slot,
+                                             unsigned o0, unsigned o1, float *acc)
+{
+    for (unsigned o = o0; o < o1; o += 4) {
+        unsigned idx[4];
+        float x[4][S];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            idx[u] = out_slot[(o + u < o1) ? o + u : o];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            load_rowp<S, SP>(rows + (size_t)idx[u] * SP, x[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (o + u < o1)
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    acc[k] = acc[k] + x[u][k];
+    }
+}
+
 template <int S, int STEP0>
 __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
 {
@@ -219,15 +243,8 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
                             T[k] = T[k] * mk;
                     }
                 }
-                if (a.w_in) {
-                    for (unsigned o = o0; o < o1; ++o) {
-                        float x[S];
-                        load_rowp<S, SP>(a.w_in + (size_t)a.out_slot[o] * SP, x);
-#pragma unroll
-                        for (int k = 0; k < S; ++k)
-                            gb[k] = gb[k] + x[k];
-                    }
-                }
+                if (a.w_in)
+                    sum_out_rows<S, SP>(a.w_in, a.out_slot, o0, o1, gb);
                 float out[S];
                 if (STEP0) {
 #pragma unroll
@@ -333,61 +350,82 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
 }
 
 /* dL/dst_0 = the step-0 partial + sum of w_0 over the node's out-slots */
+template <int S>
 __global__ void k_initial_grad(const float *__restrict__ part, const float *__restrict__ w0,
                                const unsigned *__restrict__ out_off, const unsigned *__restrict__ out_slot,
-                               float *__restrict__ out, unsigned V, unsigned S, unsigned SP)
+                               float *__restrict__ out, unsigned V)
 {
+    constexpr int SP = Stride<S>::value;
     unsigned v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V)
         return;
-    for (unsigned k = 0; k < SP; ++k) {
-        float acc = part[(size_t)v * SP + k];
-        if (k < S)
-            for (unsigned o = out_off[v]; o < out_off[v + 1]; ++o)
-                acc = acc + w0[(size_t)out_slot[o] * SP + k];
-        out[(size_t)v * SP + k] = (k < S) ? acc : 0.0f;
-    }
+    float acc[S];
+    load_rowp<S, SP>(part + (size_t)v * SP, acc);
+    sum_out_rows<S, SP>(w0, out_slot, out_off[v], out_off[v + 1], acc);
+#pragma unroll
+    for (int k = 0; k < SP; ++k)
+        out[(size_t)v * SP + k] = (k < S) ? acc[k] : 0.0f;
 }
 
-/* dL/dparam: the transpose of k_tables and k_initial.  The parent records
- * and dJ entries of up to eight parents are loaded before any is summed. */
-__global__ void k_param_grad(const float *__restrict__ jbar, const float *__restrict__ dst0,
-                             const unsigned *__restrict__ def_of, const unsigned *__restrict__ def_tb,
-                             const unsigned *__restrict__ def_slice, const unsigned *__restrict__ def_pbegin,
-                             const ParentRec *__restrict__ parents, const unsigned *__restrict__ def_prior,
-                             float *__restrict__ out, unsigned num_params, unsigned S, unsigned SP)
+/* G[pr][k][j] = dJ of the forward slot at [j][k] + dJ of the reverse slot at
+ * [k][j]: everything a parameter entry of the definition gets through
+ * parent record pr when that parent is in state j and the child in k */
+__global__ void k_parent_grad(const float *__restrict__ jbar, const ParentRec *__restrict__ parents,
+                              float *__restrict__ G, unsigned nrec, unsigned S)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t SS = (size_t)S * S;
+    if (t >= (size_t)nrec * SS)
+        return;
+    unsigned pr = (unsigned)(t / SS), kj = (unsigned)(t % SS), k = kj / S, j = kj % S;
+    ParentRec rec = parents[pr];
+    float g = 0.0f;
+    if (j < rec.n_par) {
+        g = jbar[(size_t)rec.slot_f * SS + j * S + k];
+        if (rec.slot_r != NONE)
+            g = g + jbar[(size_t)rec.slot_r * SS + k * S + j];
+    }
+    G[t] = g;
+}
+
+/* dL/dparam: the transpose of k_tables and k_initial, from the per-parent
+ * sums G.  Parent records and G entries of up to eight parents are loaded
+ * before any is added. */
+struct DefInfo {
+    unsigned tb, slice, pbegin, pend, prior, pad0, pad1, pad2;
+};
+
+__global__ void k_param_grad(const float *__restrict__ G, const float *__restrict__ dst0,
+                             const unsigned *__restrict__ def_of, const DefInfo *__restrict__ defs,
+                             const ParentRec *__restrict__ parents, float *__restrict__ out, unsigned num_params,
+                             unsigned S, unsigned SP)
 {
     unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= num_params)
         return;
-    unsigned d = def_of[p];
-    unsigned local = p - def_tb[d], slice = def_slice[d];
-    unsigned k = local / slice, idx = local % slice;
-    unsigned i0 = def_pbegin[d], i1 = def_pbegin[d + 1];
+    DefInfo di = defs[def_of[p]];
+    unsigned local = p - di.tb;
+    unsigned k = local / di.slice, idx = local % di.slice;
     size_t SS = (size_t)S * S;
     float g = 0.0f;
-    for (unsigned base = i0; base < i1; base += 8) {
-        float f[8], r[8];
+    for (unsigned base = di.pbegin; base < di.pend; base += 8) {
+        float f[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             f[u] = 0.0f;
-            r[u] = 0.0f;
-            if (base + u < i1) {
+            if (base + u < di.pend) {
                 ParentRec pr = parents[base + u];
                 unsigned j = (idx / pr.offset) % pr.n_par;
-                f[u] = jbar[(size_t)pr.slot_f * SS + j * S + k];
-                if (pr.slot_r != NONE)
-                    r[u] = jbar[(size_t)pr.slot_r * SS + k * S + j];
+                f[u] = G[(size_t)(base + u) * SS + k * S + j];
             }
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
-            if (base + u < i1)
-                g = g + f[u] + r[u];
+            if (base + u < di.pend)
+                g = g + f[u];
     }
-    unsigned v = def_prior[d];
-    if (v != NONE)
-        g = g + dst0[(size_t)v * SP + k];
+    if (di.prior != NONE)
+        g = g + dst0[(size_t)di.prior * SP + k];
     out[p] = g;
 }
 
@@ -467,6 +505,8 @@ struct bp_diff {
     unsigned *prior_tb = nullptr;
     unsigned *out_off = nullptr, *out_slot = nullptr, *slot_edge = nullptr;
     unsigned *def_of = nullptr, *def_tb = nullptr, *def_slice = nullptr, *def_pbegin = nullptr, *def_prior = nullptr;
+    DefInfo *defs = nullptr; /* the same per definition, packed for k_param_grad */
+    float *G = nullptr;      /* [nrec][S][S] per-parent gradient sums */
     ParentRec *parents = nullptr;
 
     float *P = nullptr;     /* parameters */
@@ -518,7 +558,7 @@ extern "C" void bp_diff_free(bp_diff *x)
         (void)hipDeviceSynchronize();
     }
     diff_release_storage(x);
-    void *ps[] = {x->perm_d, x->inv_d, x->row_first_d, x->row_stride_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
+    void *ps[] = {x->defs, x->G, x->perm_d, x->inv_d, x->row_first_d, x->row_stride_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
                   x->def_pbegin, x->def_prior, x->parents, x->P, x->st0, x->gP, x->gin, x->gst[0], x->gst[1],
                   x->w[0], x->w[1], x->jbar, x->dst0p, x->dst0};
     for (void *p : ps)
@@ -648,6 +688,9 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
             prior_tb[d->perm[v]] = def_tb[state_def[v]];
             def_prior[state_def[v]] = d->perm[v];
         }
+    std::vector<DefInfo> definfo(npd);
+    for (unsigned dd = 0; dd < npd; ++dd)
+        definfo[dd] = DefInfo{def_tb[dd], def_slice[dd], def_pbegin[dd], def_pbegin[dd + 1], def_prior[dd], 0, 0, 0};
     /* softmax rows */
     x->row_first.clear();
     x->row_stride.clear();
@@ -686,6 +729,7 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
         upload_vec(&x->out_slot, out_slot) || upload_vec(&x->slot_edge, d->slot_edge) ||
         upload_vec(&x->def_of, def_of) || upload_vec(&x->def_tb, def_tb) || upload_vec(&x->def_slice, def_slice) ||
         upload_vec(&x->def_pbegin, def_pbegin) || upload_vec(&x->def_prior, def_prior) ||
+        upload_vec(&x->defs, definfo) || dmalloc(&x->G, std::max<size_t>((size_t)prs.size() * S * S, 1)) ||
         upload_vec(&x->parents, prs) || upload_vec(&x->par_def, par_def) || dmalloc(&x->P, x->num_params) || dmalloc(&x->gP, x->num_params) ||
         dmalloc(&x->st0, VS) || dmalloc(&x->gin, VS) || dmalloc(&x->gst[0], VS) || dmalloc(&x->gst[1], VS) ||
         dmalloc(&x->w[0], ES) || dmalloc(&x->w[1], ES) || dmalloc(&x->jbar, (size_t)E * S * S) ||
@@ -1008,12 +1052,32 @@ static void enqueue_backward(bp_diff *x, hipStream_t st)
     /* step 0: init */
     adj_step(x, 1, nullptr, x->st0, x->st0, gin, x->dst0p, w_in, x->w[0] == w_in ? x->w[1] : x->w[0], first, st);
     const float *w0 = (x->w[0] == w_in) ? x->w[1] : x->w[0];
-    hipLaunchKernelGGL(k_initial_grad, dim3(blocks_for(x->V)), dim3(256), 0, st, x->dst0p, w0, x->out_off,
-                       x->out_slot, x->dst0, x->V, x->S, x->SP);
+    dim3 gv(blocks_for(x->V)), b256(256);
+#define BP_IG_CASE(SS)                                                                                     \
+    case SS:                                                                                               \
+        hipLaunchKernelGGL(k_initial_grad<SS>, gv, b256, 0, st, x->dst0p, w0, x->out_off, x->out_slot,     \
+                           x->dst0, x->V);                                                                 \
+        break;
+    switch (x->S) {
+        BP_IG_CASE(1)
+        BP_IG_CASE(2)
+        BP_IG_CASE(3)
+        BP_IG_CASE(4)
+        BP_IG_CASE(5)
+        BP_IG_CASE(6)
+        BP_IG_CASE(7)
+        BP_IG_CASE(8)
+        BP_IG_CASE(9)
+        BP_IG_CASE(10)
+    }
+#undef BP_IG_CASE
+    size_t nG = (size_t)x->nrec * x->S * x->S;
+    if (nG)
+        hipLaunchKernelGGL(k_parent_grad, dim3(blocks_for(nG)), dim3(256), 0, st, x->jbar, x->parents, x->G, x->nrec,
+                           x->S);
     if (x->num_params)
-        hipLaunchKernelGGL(k_param_grad, dim3(blocks_for(x->num_params)), dim3(256), 0, st, x->jbar, x->dst0,
-                           x->def_of, x->def_tb, x->def_slice, x->def_pbegin, x->parents, x->def_prior, x->gP,
-                           x->num_params, x->S, x->SP);
+        hipLaunchKernelGGL(k_param_grad, dim3(blocks_for(x->num_params)), dim3(256), 0, st, x->G, x->dst0, x->def_of,
+                           x->defs, x->parents, x->gP, x->num_params, x->S, x->SP);
 }
 
 extern "C" int bp_diff_backward_ex(bp_diff *x, const float *grad_beliefs, float *grad_params, float *grad_tables,

@@ -330,3 +330,40 @@ def test_device_node_order_does_not_change_results(lib, case):
         lib.graph_destroy(g)
     assert same_bits(out[0][0], out[1][0]) and same_bits(out[0][1], out[1][1])
     assert np.allclose(out[0][2], out[1][2], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("edges", [0, 6])
+def test_isolated_nodes_and_chunks_without_slots(lib, orc, edges):
+    """Chunks whose nodes have no in-edges (every chunk, when the graph has
+    no edges) take no slot loads; results equal the oracle's."""
+    V = 300
+    import ctypes as C
+
+    def build():
+        g = lib.create_graph(V, max(edges, 1))
+        st = (C.c_float * 10)()
+        for v in range(V):
+            lib.graph_add_node(g, 2, f"n{v}".encode())
+            if v % 7 == 0:
+                st[0], st[1] = 0.25, 0.75
+                lib.graph_set_node_state(g, v, 2, st)
+        tab = (C.c_float * 100)()
+        for i in range(edges):
+            tab[0], tab[1], tab[10], tab[11] = 0.3 + 0.1 * i, 0.7, 0.6, 0.4 - 0.05 * i
+            lib.graph_add_edge(g, i, i + 1, 2, 2, tab)
+        lib.set_up_src_nodes_to_edges(g)
+        lib.set_up_dest_nodes_to_edges(g)
+        return g
+
+    g, h = build(), build()
+    dev = B.Device(g)
+    dev.init()
+    dev.iterate(5)
+    dev.download(g)
+    orc.init_previous_edge(h)
+    for _ in range(5):
+        orc.loopy_iteration(h)
+    assert same_bits(B.states(g), B.states(h)), bits_report(B.states(g), B.states(h))
+    dev.free()
+    lib.graph_destroy(g)
+    lib.graph_destroy(h)
