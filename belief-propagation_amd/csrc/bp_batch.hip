@@ -46,17 +46,25 @@ __global__ void k_zero_counter(unsigned *q)
         *q = 0u;
 }
 
-/* deltas[i] = sum of instance i's chunk partials, in chunk order */
-__global__ void k_batch_finish(const double *__restrict__ partials, const unsigned *__restrict__ inst_chunk,
-                               unsigned n_inst, double *__restrict__ deltas)
+/* deltas[i] = sum of instance i's chunk partials (chunks
+ * [inst_chunk[i], inst_chunk[i + 1])): one wave per instance, lane l
+ * summing chunks l, l + 64, ..., then a fixed-order wave reduction */
+__global__ __launch_bounds__(256) void k_batch_finish(const double *__restrict__ partials,
+                                                      const unsigned *__restrict__ inst_chunk, unsigned n_inst,
+                                                      double *__restrict__ deltas)
 {
-    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     if (i >= n_inst)
         return;
+    unsigned c0 = inst_chunk[i], c1 = inst_chunk[i + 1];
     double t = 0.0;
-    for (unsigned c = inst_chunk[i]; c < inst_chunk[i + 1]; ++c)
+    for (unsigned c = c0 + lane; c < c1; c += WAVE)
         t += partials[c];
-    deltas[i] = t;
+#pragma unroll
+    for (int o = WAVE / 2; o > 0; o >>= 1)
+        t += __shfl_xor(t, o, WAVE);
+    if (lane == 0)
+        deltas[i] = t;
 }
 
 inline unsigned nblocks(size_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -84,13 +92,21 @@ uint64_t hash_table(unsigned x, unsigned y, const float *T, bool transposed)
     return h;
 }
 
+/* Chunks in instance order (chunk c ends where chunk c + 1 begins; a
+ * sentinel closes the list) and the order they run in.  The instances of a
+ * group -- consecutive instances over the same node set, i.e. one network
+ * under several evidence sets -- run merged by starting node, so the
+ * group's copies of a node range run back to back and read the same tables
+ * while they are cached. */
 struct Shape {
     unsigned R = 1;
     std::vector<uint2> chunks;
     std::vector<unsigned> inst_chunk; /* [instances+1] */
+    std::vector<unsigned> order;      /* [nchunks] */
     uint2 *d_chunk = nullptr;
-    unsigned *d_inst_chunk = nullptr;
+    unsigned *d_inst_chunk = nullptr, *d_order = nullptr;
     unsigned nchunks = 0;
+    bool built = false;
 };
 
 struct UniqueTable {
@@ -110,9 +126,11 @@ struct bp_batch {
     std::vector<unsigned char> slot_xy;
     std::vector<unsigned> edge_x;  /* per slot: x of the slot's edge (for downloads) */
     std::vector<float> tabs;
+    std::vector<uint64_t> inst_sig; /* [instances] hash of the node set (node count, state counts) */
     std::vector<UniqueTable> uniq;
     std::unordered_map<uint64_t, unsigned> lookup; /* hash -> index in uniq */
     double alg_bytes = 0.0;
+    uint64_t distinct_floats = 0; /* a table and its transpose counted once */
     bool uploaded = false;
 
     /* device */
@@ -167,7 +185,8 @@ extern "C" void bp_batch_free(bp_batch *b)
         void *ps[] = {b->d_in_off, b->d_slot_src, b->d_slot_tab, b->d_queue, b->d_slot_xy, b->d_nvars, b->d_tabs,
                       b->d_states, b->d_states0, b->d_buf[0], b->d_buf[1], b->d_msg[0], b->d_msg[1],
                       b->d_partials, b->d_deltas, b->shapes[0].d_chunk, b->shapes[0].d_inst_chunk,
-                      b->shapes[1].d_chunk, b->shapes[1].d_inst_chunk};
+                      b->shapes[0].d_order, b->shapes[1].d_chunk, b->shapes[1].d_inst_chunk,
+                      b->shapes[1].d_order};
         for (void *p : ps)
             if (p)
                 (void)hipFree(p);
@@ -177,44 +196,48 @@ extern "C" void bp_batch_free(bp_batch *b)
     delete b;
 }
 
-/* index of a distinct table equal to T (x rows, y cols, MAX_STATES-strided),
- * or to its transpose (*tr = 1); appends it when new */
-static unsigned intern_table(bp_batch *b, unsigned x, unsigned y, const float *T, bool *tr)
+/* Row stride of a pooled table with y columns: rows padded with zeros to a
+ * multiple of four floats, so every row starts 16-byte aligned. */
+static inline unsigned pooled_stride(unsigned y) { return (y + 3u) & ~3u; }
+
+/* index of the pooled table equal to T (x rows, y cols, MAX_STATES-strided),
+ * appended when new.  The pool holds each table in the orientation its
+ * slots read it; a table whose transpose is already pooled (the reverse of
+ * an edge) is stored again, but counted once in the distinct floats. */
+static unsigned intern_table(bp_batch *b, unsigned x, unsigned y, const float *T)
 {
     auto same = [&](const UniqueTable &u, bool transposed) {
-        unsigned rows = transposed ? y : x, cols = transposed ? x : y;
+        unsigned rows = transposed ? y : x, cols = transposed ? x : y, st = pooled_stride(cols);
         if (u.x != rows || u.y != cols)
             return false;
         for (unsigned i = 0; i < rows; ++i)
             for (unsigned j = 0; j < cols; ++j) {
                 float v = transposed ? T[j * MAX_STATES + i] : T[i * MAX_STATES + j];
-                if (std::memcmp(&v, &b->tabs[u.off + i * cols + j], 4) != 0)
+                if (std::memcmp(&v, &b->tabs[u.off + i * st + j], 4) != 0)
                     return false;
             }
         return true;
     };
     uint64_t h = hash_table(x, y, T, false);
     auto it = b->lookup.find(h);
-    if (it != b->lookup.end() && same(b->uniq[it->second], false)) {
-        *tr = false;
+    if (it != b->lookup.end() && same(b->uniq[it->second], false))
         return it->second;
-    }
     uint64_t ht = hash_table(x, y, T, true);
     auto jt = b->lookup.find(ht);
-    if (jt != b->lookup.end() && same(b->uniq[jt->second], true)) {
-        *tr = true;
-        return jt->second;
-    }
+    bool transpose_pooled = jt != b->lookup.end() && same(b->uniq[jt->second], true);
+    unsigned st = pooled_stride(y);
     UniqueTable u{(unsigned)b->tabs.size(), x, y};
     for (unsigned i = 0; i < x; ++i)
-        for (unsigned j = 0; j < y; ++j)
-            b->tabs.push_back(T[i * MAX_STATES + j]);
+        for (unsigned j = 0; j < st; ++j)
+            b->tabs.push_back(j < y ? T[i * MAX_STATES + j] : 0.0f);
     b->uniq.push_back(u);
     unsigned idx = (unsigned)b->uniq.size() - 1;
     if (it == b->lookup.end())
         b->lookup.emplace(h, idx);
-    b->alg_bytes += 4.0 * x * y;
-    *tr = false;
+    if (!transpose_pooled) {
+        b->alg_bytes += 4.0 * x * y;
+        b->distinct_floats += (uint64_t)x * y;
+    }
     return idx;
 }
 
@@ -270,6 +293,14 @@ extern "C" int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index)
         }
     }
     unsigned n0 = b->node_off.back(), s0 = b->slot_off.back();
+    uint64_t sig = 1469598103934665603ull;
+    auto mix = [&sig](uint32_t u) {
+        sig ^= u;
+        sig *= 1099511628211ull;
+    };
+    mix(V);
+    for (unsigned v = 0; v < V; ++v)
+        mix(g->node_num_vars[v]);
     for (unsigned v = 0; v < V; ++v) {
         unsigned n = g->node_num_vars[v];
         b->S = std::max(b->S, n);
@@ -282,15 +313,14 @@ extern "C" int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index)
     for (unsigned k = 0; k < E; ++k) {
         unsigned e = g->dest_nodes_to_edges_edge_list[k];
         unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
-        bool tr = false;
-        unsigned u = intern_table(b, x, y, g->edges_joint_probabilities + (size_t)e * MAX_STATES * MAX_STATES, &tr);
-        unsigned off = b->uniq[u].off;
-        if (off >= 0x80000000u) {
-            bp_set_error("bp_batch_add: more than 2^31 distinct table entries");
+        if (b->tabs.size() + (size_t)x * pooled_stride(y) >= 0xffffffffull) {
+            bp_set_error("bp_batch_add: the table pool exceeds 2^32 floats");
             return -1;
         }
+        unsigned u = intern_table(b, x, y, g->edges_joint_probabilities + (size_t)e * MAX_STATES * MAX_STATES);
+        unsigned off = b->uniq[u].off;
         b->slot_src.push_back(n0 + g->edges_src_index[e]);
-        b->slot_tab.push_back(off | (tr ? 0x80000000u : 0u));
+        b->slot_tab.push_back(off);
         b->slot_xy.push_back((unsigned char)(x | (y << 4)));
         b->slot_edge.push_back(e);
         b->edge_x.push_back(x);
@@ -298,12 +328,19 @@ extern "C" int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index)
     }
     b->node_off.push_back(n0 + V);
     b->slot_off.push_back(s0 + E);
+    b->inst_sig.push_back(sig);
     if (index)
         *index = (unsigned)b->node_off.size() - 2;
     return 0;
 }
 
-/* chunks per instance, never across instances (cap = 64 x slots per lane) */
+/* chunks per instance, never across instances (cap = 64 x slots per lane);
+ * then the order they run in: each group of up to MAX_GROUP consecutive
+ * instances with equal node-set signatures merged by (first node within
+ * the instance, instance).  The order only decides which chunks run
+ * together: every order gives the same results. */
+constexpr unsigned MAX_GROUP = 16;
+
 static void build_shape(bp_batch *b, Shape &sh, unsigned R)
 {
     sh.R = R;
@@ -327,6 +364,23 @@ static void build_shape(bp_batch *b, Shape &sh, unsigned R)
     }
     sh.nchunks = (unsigned)sh.chunks.size();
     sh.chunks.push_back(make_uint2(b->V, b->E));
+    sh.order.clear();
+    sh.order.reserve(sh.nchunks);
+    std::vector<std::pair<uint64_t, unsigned>> keys;
+    for (unsigned i = 0; i < ninst;) {
+        unsigned j = i + 1;
+        while (j < ninst && j - i < MAX_GROUP && b->inst_sig[j] == b->inst_sig[i])
+            ++j;
+        keys.clear();
+        for (unsigned k = i; k < j; ++k)
+            for (unsigned c = sh.inst_chunk[k]; c < sh.inst_chunk[k + 1]; ++c)
+                keys.emplace_back(((uint64_t)(sh.chunks[c].x - b->node_off[k]) << 32) | (k - i), c);
+        std::sort(keys.begin(), keys.end());
+        for (const auto &kv : keys)
+            sh.order.push_back(kv.second);
+        i = j;
+    }
+    sh.built = true;
 }
 
 template <typename T> static int up(T **dst, const T *src, size_t n)
@@ -394,10 +448,11 @@ extern "C" int bp_batch_upload(bp_batch *b, int device)
         return -1;
     for (int k = 0; k < 2; ++k) {
         Shape &sh = b->shapes[k];
-        if (sh.chunks.empty())
+        if (!sh.built)
             continue;
         if (up(&sh.d_chunk, sh.chunks.data(), sh.chunks.size()) ||
-            up(&sh.d_inst_chunk, sh.inst_chunk.data(), sh.inst_chunk.size()))
+            up(&sh.d_inst_chunk, sh.inst_chunk.data(), sh.inst_chunk.size()) ||
+            up(&sh.d_order, sh.order.data(), sh.order.size()))
             return -1;
     }
     HIP_TRY(hipMemcpy(b->d_states, b->d_states0, VS * sizeof(float), hipMemcpyDeviceToDevice));
@@ -474,6 +529,7 @@ static LoopyArgs batch_args(bp_batch *b)
     a.msg_in = b->d_msg[b->latest];
     a.msg_out = b->d_msg[b->latest ^ 1];
     a.chunk = sh.d_chunk;
+    a.order = sh.d_order;
     a.nchunks = sh.nchunks;
     a.partials = nullptr;
     a.stop = nullptr;
@@ -621,8 +677,8 @@ extern "C" int bp_batch_iterate(bp_batch *b, unsigned int n, double *deltas)
             a.msg_in = b->d_msg[lat];
             a.msg_out = b->d_msg[lat ^ 1];
             launch_batch(b, ITER, a, b->stream);
-            hipLaunchKernelGGL(k_batch_finish, dim3(nblocks(b->n_inst)), dim3(256), 0, b->stream, b->d_partials,
-                               sh.d_inst_chunk, b->n_inst, b->d_deltas + (size_t)i * b->n_inst);
+            hipLaunchKernelGGL(k_batch_finish, dim3(nblocks(b->n_inst, WAVES_PER_BLOCK)), dim3(256), 0, b->stream,
+                               b->d_partials, sh.d_inst_chunk, b->n_inst, b->d_deltas + (size_t)i * b->n_inst);
             lat ^= 1;
             bc ^= 1;
         }
@@ -718,10 +774,7 @@ extern "C" int bp_batch_info(const bp_batch *b, uint64_t *info)
     info[1] = b->node_off.back();
     info[2] = b->slot_off.back();
     info[3] = b->S;
-    uint64_t tf = 0;
-    for (const auto &u : b->uniq)
-        tf += (uint64_t)u.x * u.y;
-    info[4] = tf;
+    info[4] = b->distinct_floats;
     info[5] = b->uploaded ? cur_shape(b).nchunks : 0;
     info[6] = (uint64_t)b->alg_bytes;
     return 0;

@@ -89,12 +89,14 @@ struct LoopyArgs {
     const float *msg_in;         /* [E][SP] old messages (ITER) / messages (PREP) */
     float *msg_out;              /* [E][SP] new messages (ITER, INIT) */
     const uint2 *chunk;          /* [nchunks+1] (first node, first slot) of each chunk */
+    const unsigned *order;       /* non-null: [nchunks] the p-th chunk taken is chunk order[p] */
     unsigned nchunks;
     double *partials;            /* [G * waves per block] this launch's per-wave delta partials (ITER) */
     const unsigned *stop;        /* non-null: skip the launch once *stop != 0 */
-    /* packed tables (batched networks): slot s's table starts at
-     * tabs + (slot_tab[s] & 0x7fffffff), row-major with the slot's x rows and
-     * y columns, or stored transposed (y rows, x columns) when bit 31 is set */
+    /* pooled tables (batched networks): slot s's table starts at
+     * tabs + slot_tab[s], row-major with the slot's x rows of y entries,
+     * each row padded with zeros to a multiple of four floats (16-byte
+     * aligned rows) */
     const unsigned *slot_tab;
     const float *tabs;
     double *chunk_partials;      /* non-null: one delta partial per chunk instead of per block */
@@ -214,6 +216,24 @@ __device__ __forceinline__ void send(const float *__restrict__ Jrow, const float
         m[i] = m[i] / sum;
 }
 
+/* Row i of a pooled table (rows of y entries padded to a multiple of four)
+ * into r[0..S): aligned four-float loads, zero beyond y. */
+template <int S>
+__device__ __forceinline__ void load_pooled_row(const float *__restrict__ T, unsigned i, unsigned y, float *r)
+{
+    const float *row = T + (size_t)i * ((y + 3u) & ~3u);
+#pragma unroll
+    for (int q = 0; q < (S + 3) / 4; ++q) {
+        v4f v = {0.0f, 0.0f, 0.0f, 0.0f};
+        if ((unsigned)(4 * q) < y)
+            v = *reinterpret_cast<const v4f *>(row + 4 * q);
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (4 * q + c < S)
+                r[4 * q + c] = v[c];
+    }
+}
+
 /* The table of slot s as J[i*S + j] (zero beyond x rows and y columns). */
 template <int S, bool PACKED>
 __device__ __forceinline__ void load_table(const LoopyArgs &a, unsigned s, unsigned xy, float *J)
@@ -221,14 +241,16 @@ __device__ __forceinline__ void load_table(const LoopyArgs &a, unsigned s, unsig
     if constexpr (!PACKED) {
         load_row_nt<S * S>(a.J + (size_t)s * S * S, J);
     } else {
-        unsigned t = a.slot_tab[s], x = xy & 15u, y = xy >> 4;
-        const float *T = a.tabs + (t & 0x7fffffffu);
-        bool tr = (t >> 31) != 0;
+        unsigned x = xy & 15u, y = xy >> 4;
+        const float *T = a.tabs + a.slot_tab[s];
 #pragma unroll
-        for (int i = 0; i < S; ++i)
+        for (int i = 0; i < S; ++i) {
+            float r[S];
+            load_pooled_row<S>(T, i, (unsigned)i < x ? y : 0u, r);
 #pragma unroll
             for (int j = 0; j < S; ++j)
-                J[i * S + j] = ((unsigned)i < x && (unsigned)j < y) ? T[tr ? j * x + i : i * y + j] : 0.0f;
+                J[i * S + j] = r[j];
+        }
     }
 }
 
@@ -239,27 +261,24 @@ template <int S, bool MIXED, bool PACKED>
 __device__ __forceinline__ void send_from_memory(const LoopyArgs &a, unsigned s, unsigned xy, const float *b, float *m)
 {
     unsigned x = MIXED ? (xy & 15u) : (unsigned)S, y = MIXED ? (xy >> 4) : (unsigned)S;
-    const float *T;
-    bool tr = false;
-    if constexpr (PACKED) {
-        unsigned t = a.slot_tab[s];
-        T = a.tabs + (t & 0x7fffffffu);
-        tr = (t >> 31) != 0;
-    } else {
-        T = a.J + (size_t)s * S * S;
-    }
-    /* row i starts at T + i*y (or T + i stored transposed), entries step 1 (or x) apart */
-    const unsigned rstep = PACKED ? (tr ? 1u : y) : (unsigned)S, cstep = (PACKED && tr) ? x : 1u;
+    const float *T = PACKED ? a.tabs + a.slot_tab[s] : a.J + (size_t)s * S * S;
     float sum = 0.0f;
 #pragma unroll
     for (int i = 0; i < S; ++i) {
         float acc = 0.0f;
         if (!MIXED || (unsigned)i < x) {
-            const float *row = T + (size_t)i * rstep;
+            float row[S];
+            if constexpr (PACKED) {
+                load_pooled_row<S>(T, i, y, row);
+            } else {
+#pragma unroll
+                for (int j = 0; j < S; ++j)
+                    row[j] = T[(size_t)i * S + j];
+            }
 #pragma unroll
             for (int j = 0; j < S; ++j)
                 if (!MIXED || (unsigned)j < y)
-                    acc = acc + row[(size_t)j * cstep] * b[j];
+                    acc = acc + row[j] * b[j];
             sum = sum + acc;
         }
         m[i] = acc;
@@ -682,8 +701,43 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
         n = MIXED ? a.nvars[v] : S;
     }
     wave_sync();
-#pragma unroll 1
-    for (int p = 0; p < L; ++p) {
+    /* The table entries of pass p that this lane multiplies: rows
+     * i = k, k + L, ... and columns j < y (zero elsewhere).  They are all
+     * loaded before the first product, so the pass waits on memory once
+     * rather than once per row (1.5x on mixed 2-8 state batches; loading
+     * the next pass's entries during this pass as well was no faster, the
+     * extra registers costing occupancy). */
+    auto load_pass = [&](int p, float (&tv)[NR][S]) {
+        unsigned q = p * G + g, s = sb + q;
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+                tv[r][j] = 0.0f;
+        if (MODE == PREP || s >= se)
+            return;
+        unsigned x = S, y = S;
+        if (MIXED) {
+            unsigned xy = stage_u[q];
+            x = xy & 15u;
+            y = xy >> 4;
+        }
+        const float *T = PACKED ? a.tabs + stage_u[W + q] : a.J + (size_t)s * S * S;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            unsigned i = k + r * L;
+            if (i < (unsigned)S && (!MIXED || i < x)) {
+                if constexpr (PACKED) {
+                    load_pooled_row<S>(T, i, y, tv[r]);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < S; ++j)
+                        tv[r][j] = T[(size_t)i * S + j];
+                }
+            }
+        }
+    };
+    auto run_pass = [&](int p, const float (&tv)[NR][S]) {
         unsigned q = p * G + g, s = sb + q;
         bool act = s < se;
         unsigned x = S, y = S;
@@ -709,26 +763,15 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
 #pragma unroll
                 for (int j = 0; j < S; ++j)
                     b[j] = stage_f[j * W + q];
-                const float *T;
-                bool tr = false;
-                if constexpr (PACKED) {
-                    unsigned t = stage_u[W + q];
-                    T = a.tabs + (t & 0x7fffffffu);
-                    tr = (t >> 31) != 0;
-                } else {
-                    T = a.J + (size_t)s * S * S;
-                }
-                const unsigned rstep = PACKED ? (tr ? 1u : y) : (unsigned)S, cstep = (PACKED && tr) ? x : 1u;
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
                     unsigned i = k + r * L;
                     float sacc = 0.0f;
                     if (i < (unsigned)S && (!MIXED || i < x)) {
-                        const float *row = T + (size_t)i * rstep;
 #pragma unroll
                         for (int j = 0; j < S; ++j)
                             if (!MIXED || (unsigned)j < y)
-                                sacc = sacc + row[(size_t)j * cstep] * b[j];
+                                sacc = sacc + tv[r][j] * b[j];
                     }
                     acc[r] = sacc;
                 }
@@ -773,6 +816,12 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                     a.msg_out[(size_t)s * SP + i] = (i < (unsigned)S) ? m[r] : 0.0f;
             }
         }
+    };
+#pragma unroll 1
+    for (int p = 0; p < L; ++p) {
+        float tv[NR][S];
+        load_pass(p, tv);
+        run_pass(p, tv);
     }
     wave_sync();
     if (v < ne)
@@ -869,7 +918,7 @@ __device__ __forceinline__ void run_chunk(const LoopyArgs &a, unsigned c, float 
 }
 
 /* per-chunk delta partial (batched networks: chunks never span two
- * networks, so per-network deltas are sums over chunk ranges) */
+ * networks, so per-network deltas are sums over each network's chunks) */
 __device__ __forceinline__ void flush_chunk_partial(const LoopyArgs &a, unsigned c, unsigned lane, double &dacc)
 {
     double d = wave_sum(dacc);
@@ -892,7 +941,8 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
     double dacc = 0.0;
     const bool per_chunk = (MODE == ITER || MODE == EDGE) && a.chunk_partials != nullptr;
     if (a.queue) {
-        /* work queue: each wave takes blocks of qblock consecutive chunks */
+        /* work queue: each wave takes blocks of qblock consecutive positions
+         * of the chunk order */
         for (;;) {
             unsigned base = 0;
             if (lane == 0)
@@ -901,21 +951,23 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
             if (base >= a.nchunks)
                 break;
             unsigned end = base + a.qblock < a.nchunks ? base + a.qblock : a.nchunks;
-            for (unsigned c = base; c < end; ++c) {
+            for (unsigned p = base; p < end; ++p) {
+                unsigned c = a.order ? a.order[p] : p;
                 run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], stage_f[wave], stage_u[wave], lane, dacc);
                 if (per_chunk)
                     flush_chunk_partial(a, c, lane, dacc);
             }
         }
     } else {
-        /* wave w of the grid takes the contiguous chunk range
-         * [w * nchunks / W, (w + 1) * nchunks / W), W = waves in the grid;
+        /* wave w of the grid takes positions [w * nchunks / W,
+         * (w + 1) * nchunks / W) of the chunk order, W = waves in the grid;
          * with the XCD remap each XCD's waves cover one contiguous range */
         unsigned W = gridDim.x * WAVES_PER_BLOCK;
         unsigned w = xcd_block(blockIdx.x, gridDim.x) * WAVES_PER_BLOCK + wave;
         unsigned c0 = (unsigned)(((unsigned long long)w * a.nchunks) / W);
         unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / W);
-        for (unsigned c = c0; c < c1; ++c) {
+        for (unsigned p = c0; p < c1; ++p) {
+            unsigned c = a.order ? a.order[p] : p;
             run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], stage_f[wave], stage_u[wave], lane, dacc);
             if (per_chunk)
                 flush_chunk_partial(a, c, lane, dacc);

@@ -92,12 +92,13 @@ def bench_adjoint(K: int = 30, c: int = 5, reps: int = 10) -> dict:
     return out
 
 
-def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: int = 3) -> dict:
+def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: int = 3, plain_leg: bool = True) -> dict:
     """The batched workload (512 DAGs, node counts log-uniform in 200-50k,
     in-degree 0-4, states 2-8, 8 evidence sets of 3% each, 30 iterations):
     one batch (hipGraph-captured launches, tuned shape) against the plain
     path run network after network (init + K iterations, each finished
-    before the next starts)."""
+    before the next starts).  plain_leg=False skips the plain comparison
+    (for profiling the batch kernels alone)."""
     from workloads import batched_networks
 
     lib = B.load()
@@ -106,7 +107,8 @@ def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: i
     plain = []
     for i, e, g in batched_networks(count=count, evidence_sets=evidence_sets):
         batch.add(g)
-        plain.append(B.Device(g, 0))
+        if plain_leg:
+            plain.append(B.Device(g, 0))
         lib.graph_destroy(g)
     build_s = time.perf_counter() - t0
     batch.upload(0)
@@ -139,7 +141,8 @@ def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: i
     t_plain = time.perf_counter() - t0
     out = {"workload": f"{count} networks x {evidence_sets} evidence sets: {info['V']} nodes, {info['E']} message "
                        f"edges, {K} iterations", "instances": int(info["instances"]),
-           "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3, "speedup": t_plain / t_batch,
+           "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3 if plain_leg else None,
+           "speedup": t_plain / t_batch if plain_leg else None,
            "shape": {"slots_per_lane": shape[0], "lanes_per_slot": shape[1], "queue_block": shape[2]},
            "iteration_kernel_ms": kern_ms,
            "iteration_achieved_GBs": info["alg_bytes"] / (kern_ms * 1e-3) / 1e9,
@@ -160,6 +163,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="adjoint,batched",
                     help="comma list of the further forms measured in the same run (adjoint, batched; '' for none)")
+    ap.add_argument("--batch-nets", type=int, default=512, help="networks in the batched form (8 evidence sets each)")
+    ap.add_argument("--no-batch-plain", action="store_true",
+                    help="batched form without its plain per-network comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -252,7 +258,7 @@ def main():
     if "adjoint" in wanted:
         forms["adjoint"] = bench_adjoint()
     if "batched" in wanted:
-        forms["batched"] = bench_batched()
+        forms["batched"] = bench_batched(count=args.batch_nets, plain_leg=not args.no_batch_plain)
 
     out = {
         "metric": "edge message-updates/s at 1M edges (loopy BP, 20 iterations + FP64 delta)",

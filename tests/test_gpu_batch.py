@@ -131,3 +131,30 @@ def test_instance_matches_oracle(lib, orc):
     b.free()
     for g in graphs:
         lib.graph_destroy(g)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 0), (1, 4, 4)])
+def test_interleaved_evidence_groups_match_plain(lib, shape):
+    """20 evidence sets per network: the batch runs each network's copies
+    merged by node range, in groups of at most 16 (here 16 + 4); every
+    instance still equals its plain run, deltas included."""
+    n, sets = 12, 20
+    b, graphs = build(3, sets, 200, 900)
+    b.upload(0)
+    b.set_config(*shape)
+    b.init()
+    bd = b.iterate(n, want_deltas=True)
+    for i, g in enumerate(graphs):
+        b.download(i, g)
+    got = [copy_graph_results(lib, g) for g in graphs]
+    for g in graphs:
+        lib.graph_destroy(g)
+    for i, e, g in batched_networks(count=3, evidence_sets=sets, lo=200, hi=900):
+        k = i * sets + e
+        pd = plain_run(g, n)
+        st, mp, mc = copy_graph_results(lib, g)
+        lib.graph_destroy(g)
+        assert same_bits(got[k][0], st), f"instance {k}: {bits_report(got[k][0], st)}"
+        assert same_bits(got[k][1], mp) and same_bits(got[k][2], mc), f"instance {k} messages"
+        assert np.allclose(bd[:, k], pd, rtol=1e-12, atol=1e-300), f"instance {k} deltas"
+    b.free()
