@@ -127,6 +127,8 @@ struct bp_batch {
     std::vector<unsigned> edge_x;  /* per slot: x of the slot's edge (for downloads) */
     std::vector<float> tabs;
     std::vector<uint64_t> inst_sig; /* [instances] hash of the node set (node count, state counts) */
+    std::vector<unsigned> node_inv; /* [V] graph node (within its instance) at each device node */
+    std::vector<unsigned> last_inv; /* the device order of the last instance added */
     std::vector<UniqueTable> uniq;
     std::unordered_map<uint64_t, unsigned> lookup; /* hash -> index in uniq */
     double alg_bytes = 0.0;
@@ -301,30 +303,50 @@ extern "C" int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index)
     mix(V);
     for (unsigned v = 0; v < V; ++v)
         mix(g->node_num_vars[v]);
-    for (unsigned v = 0; v < V; ++v) {
+    /* device node order: Cuthill-McKee (or the graph's order, per
+     * bp_set_node_order), computed once per run of instances over the same
+     * node set so that their node ranges keep matching */
+    bool same_set = !b->inst_sig.empty() && b->inst_sig.back() == sig && b->last_inv.size() == V;
+    if (!same_set) {
+        if (bp_node_order_mode() && V > 1) {
+            bp_cm_order(g, b->last_inv);
+        } else {
+            b->last_inv.resize(V);
+            for (unsigned v = 0; v < V; ++v)
+                b->last_inv[v] = v;
+        }
+    }
+    const std::vector<unsigned> &inv = b->last_inv;
+    std::vector<unsigned> perm(V);
+    for (unsigned i = 0; i < V; ++i)
+        perm[inv[i]] = i;
+    unsigned slots = 0;
+    for (unsigned i = 0; i < V; ++i) {
+        unsigned v = inv[i];
         unsigned n = g->node_num_vars[v];
         b->S = std::max(b->S, n);
         b->nvars.push_back((unsigned char)n);
+        b->node_inv.push_back(v);
         for (unsigned k = 0; k < MAX_STATES; ++k)
             b->st_stage.push_back(k < n ? g->node_states[(size_t)v * MAX_STATES + k] : 0.0f);
-        b->in_off.push_back(s0 + g->dest_nodes_to_edges_node_list[v]);
+        b->in_off.push_back(s0 + slots);
         b->alg_bytes += 12.0 * n + 4.0;
-    }
-    for (unsigned k = 0; k < E; ++k) {
-        unsigned e = g->dest_nodes_to_edges_edge_list[k];
-        unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
-        if (b->tabs.size() + (size_t)x * pooled_stride(y) >= 0xffffffffull) {
-            bp_set_error("bp_batch_add: the table pool exceeds 2^32 floats");
-            return -1;
+        for (unsigned k = g->dest_nodes_to_edges_node_list[v]; k < bp_csr_end(g, g->dest_nodes_to_edges_node_list, v);
+             ++k, ++slots) {
+            unsigned e = g->dest_nodes_to_edges_edge_list[k];
+            unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
+            if (b->tabs.size() + (size_t)x * pooled_stride(y) >= 0xffffffffull) {
+                bp_set_error("bp_batch_add: the table pool exceeds 2^32 floats");
+                return -1;
+            }
+            unsigned u = intern_table(b, x, y, g->edges_joint_probabilities + (size_t)e * MAX_STATES * MAX_STATES);
+            b->slot_src.push_back(n0 + perm[g->edges_src_index[e]]);
+            b->slot_tab.push_back(b->uniq[u].off);
+            b->slot_xy.push_back((unsigned char)(x | (y << 4)));
+            b->slot_edge.push_back(e);
+            b->edge_x.push_back(x);
+            b->alg_bytes += 4.0 * (2.0 * x + y + 1.0);
         }
-        unsigned u = intern_table(b, x, y, g->edges_joint_probabilities + (size_t)e * MAX_STATES * MAX_STATES);
-        unsigned off = b->uniq[u].off;
-        b->slot_src.push_back(n0 + g->edges_src_index[e]);
-        b->slot_tab.push_back(off);
-        b->slot_xy.push_back((unsigned char)(x | (y << 4)));
-        b->slot_edge.push_back(e);
-        b->edge_x.push_back(x);
-        b->alg_bytes += 4.0 * (2.0 * x + y + 1.0);
     }
     b->node_off.push_back(n0 + V);
     b->slot_off.push_back(s0 + E);
@@ -464,6 +486,7 @@ extern "C" int bp_batch_upload(bp_batch *b, int device)
     std::vector<float>().swap(b->st_stage);
     std::vector<unsigned>().swap(b->slot_src);
     std::vector<unsigned>().swap(b->slot_tab);
+    std::vector<unsigned>().swap(b->last_inv);
     std::vector<unsigned char>().swap(b->slot_xy);
     std::vector<float>().swap(b->tabs);
     b->lookup.clear();
@@ -745,9 +768,11 @@ extern "C" int bp_batch_download(bp_batch *b, unsigned int index, Graph_t g)
     HIP_TRY(hipMemcpyAsync(m1.data(), b->d_msg[b->latest ^ 1] + (size_t)s0 * SP, m1.size() * 4,
                            hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
-    for (unsigned v = 0; v < V; ++v)
-        for (unsigned k = 0; k < b->nvars[n0 + v]; ++k)
-            g->node_states[(size_t)v * MAX_STATES + k] = st[(size_t)v * SP + k];
+    for (unsigned i = 0; i < V; ++i) {
+        unsigned v = b->node_inv[n0 + i];
+        for (unsigned k = 0; k < b->nvars[n0 + i]; ++k)
+            g->node_states[(size_t)v * MAX_STATES + k] = st[(size_t)i * SP + k];
+    }
     if (b->swaps & 1u) {
         float **t = g->previous_edge_messages;
         g->previous_edge_messages = g->current_edge_messages;

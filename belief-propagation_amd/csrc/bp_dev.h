@@ -95,6 +95,12 @@ bpk::LoopyArgs bp_make_args(bp_dev *d);
 void bp_launch_loopy(bp_dev *d, int mode, const bpk::LoopyArgs &a, hipStream_t st, const hipEvent_t *ev = nullptr);
 int bp_upload_graph(Graph_t g, int device, bp_dev **out);
 
+/* Device node numbering: bp_node_order_mode() is what bp_set_node_order
+ * chose (1: Cuthill-McKee, 0: the graph's own order); bp_cm_order fills
+ * order[i] = the graph node placed at device position i. */
+int bp_node_order_mode();
+void bp_cm_order(Graph_t g, std::vector<unsigned> &order);
+
 /* tree BP (bp_tree.hip) */
 int bp_tree_plan_build(bp_dev *d, Graph_t g);
 void bp_tree_plan_free(bp_tree_plan *p);

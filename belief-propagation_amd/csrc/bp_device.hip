@@ -178,7 +178,9 @@ extern "C" int bp_set_node_order(int mode)
     return prev;
 }
 
-static void cm_order(Graph_t g, std::vector<unsigned> &order)
+int bp_node_order_mode() { return g_node_order; }
+
+void bp_cm_order(Graph_t g, std::vector<unsigned> &order)
 {
     unsigned V = g->current_num_vertices, E = g->current_num_edges;
     std::vector<unsigned> off(V + 1, 0), adj(2 * (size_t)E);
@@ -440,7 +442,7 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
     d->V = V;
     d->E = E;
     if (g_node_order && V > 1) {
-        cm_order(g, d->inv);
+        bp_cm_order(g, d->inv);
     } else {
         d->inv.resize(V);
         for (unsigned v = 0; v < V; ++v)

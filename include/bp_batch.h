@@ -12,9 +12,14 @@
  * bp_loopy_iterate): networks never share nodes, and every node's
  * arithmetic is the same.
  *
- * Tables are stored once per distinct content: an edge table equal to an
- * earlier one, or to an earlier one's transpose (the reverse edges the
- * readers emit, and the same arcs under other evidence sets), points at it.
+ * Tables are pooled by content: an edge table equal to an earlier one (the
+ * same arc under another evidence set) points at it.  The pool keeps each
+ * table in the orientation its edges read it -- a reverse edge's table, the
+ * transpose of its forward edge's, is a second entry -- with rows padded to
+ * four-float multiples so the kernels read them with aligned vector loads.
+ * Instances over the same nodes (one network under several evidence sets)
+ * are scheduled together, node range by node range, so their shared tables
+ * are read while cached.
  *
  * The kernel's tile shape (slots per lane, i.e. edges per thread and chunk
  * size; lanes per slot, i.e. threads per edge) and scheduling (static
@@ -83,7 +88,8 @@ int bp_batch_sync(bp_batch *b);
 /* info[0]=instances, info[1]=nodes, info[2]=message edges, info[3]=largest
  * state count, info[4]=distinct table floats, info[5]=chunks of the current
  * shape, info[6]=algorithmic bytes of one iteration (see
- * bp_loopy_algorithmic_bytes; tables counted once per distinct table) */
+ * bp_loopy_algorithmic_bytes; tables counted once per distinct table, a
+ * table and its transpose being one); info[4] counts the same way */
 int bp_batch_info(const bp_batch *b, uint64_t *info);
 
 /* Times n iterations of the current shape with HIP events around the

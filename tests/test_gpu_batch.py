@@ -158,3 +158,29 @@ def test_interleaved_evidence_groups_match_plain(lib, shape):
         assert same_bits(got[k][1], mp) and same_bits(got[k][2], mc), f"instance {k} messages"
         assert np.allclose(bd[:, k], pd, rtol=1e-12, atol=1e-300), f"instance {k} deltas"
     b.free()
+
+
+def test_device_node_order_does_not_change_batch_results(lib):
+    """Instances are renumbered on the device (Cuthill-McKee, once per node
+    set); with the graphs' own order the downloads are the same bits."""
+    n = 15
+    out = {}
+    for mode in (0, 1):
+        prev = lib.bp_set_node_order(mode)
+        try:
+            b, graphs = build(4, 3, 200, 1500)
+        finally:
+            lib.bp_set_node_order(prev)
+        b.upload(0)
+        b.init()
+        d = b.iterate(n, want_deltas=True)
+        res = []
+        for i, g in enumerate(graphs):
+            b.download(i, g)
+            res.append(copy_graph_results(lib, g))
+            lib.graph_destroy(g)
+        b.free()
+        out[mode] = (res, d)
+    for k, (a, c) in enumerate(zip(out[0][0], out[1][0])):
+        assert all(same_bits(x, y) for x, y in zip(a, c)), f"instance {k}"
+    assert np.allclose(out[0][1], out[1][1], rtol=1e-12, atol=1e-300)
