@@ -101,6 +101,11 @@ int bp_upload_graph(Graph_t g, int device, bp_dev **out);
 int bp_node_order_mode();
 void bp_cm_order(Graph_t g, std::vector<unsigned> &order);
 
+/* Chunks over the slot ranges in_off[V+1]: consecutive nodes with at most
+ * cap in-slots and cap nodes, or one node with more; entry c is chunk c's
+ * (first node, first slot), and a sentinel (V, in_off[V]) closes the list. */
+void bp_build_chunks(const std::vector<unsigned> &in_off, unsigned V, unsigned cap, std::vector<uint2> &chunks);
+
 /* tree BP (bp_tree.hip) */
 int bp_tree_plan_build(bp_dev *d, Graph_t g);
 void bp_tree_plan_free(bp_tree_plan *p);

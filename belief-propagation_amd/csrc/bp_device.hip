@@ -142,7 +142,7 @@ extern "C" int bp_device_count(void)
 /* Chunks: consecutive nodes with at most cap in-slots and cap nodes (cap =
  * 64 x slots per lane), or one node with more in-slots.  Each entry holds
  * the chunk's first node and first slot. */
-static void build_chunks(const std::vector<unsigned> &in_off, unsigned V, unsigned cap, std::vector<uint2> &chunks)
+void bp_build_chunks(const std::vector<unsigned> &in_off, unsigned V, unsigned cap, std::vector<uint2> &chunks)
 {
     chunks.clear();
     unsigned v = 0;
@@ -490,7 +490,7 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
         slot_xy[s] = (unsigned char)(g->edges_x_dim[e] | (g->edges_y_dim[e] << 4));
     }
     std::vector<uint2> chunks;
-    build_chunks(in_off, V, WAVE * (S <= 4 ? (unsigned)BP_SLOTS_SMALL : 1u), chunks);
+    bp_build_chunks(in_off, V, WAVE * (S <= 4 ? (unsigned)BP_SLOTS_SMALL : 1u), chunks);
     d->nchunks = (unsigned)chunks.size() - 1;
     /* one chunk per wave; a multiple of 8 blocks so each XCD gets one range */
     unsigned want = (d->nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;

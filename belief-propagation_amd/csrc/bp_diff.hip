@@ -14,8 +14,8 @@
  * iteration of src/graph/graph.c:1121-1183 from init_previous_edge
  * (graph.c:725-751), writing every iterate to the buffer it is given.
  *
- * Adjoint of one iteration t (k_adjoint, one launch, same chunks as the
- * forward).  In exact arithmetic, with N(x) = x / sum(x):
+ * Adjoint of one iteration t (k_adjoint, one launch over chunks of its own:
+ * consecutive nodes with at most 64 x R in-slots).  In exact arithmetic, with N(x) = x / sum(x):
  *   m_t[s]  = N(z_s), z_s = J_s . buf_{t-1}[u],  u = src(s)
  *   T_t[v]  = prod_{s in in(v)} m_t[s]
  *   st_t[v] = N(a_v), a_v = st_{t-1}[v] (.) T_t[v]   (a_v = st_{t-1}[v] without in-edges)
@@ -45,6 +45,11 @@
 #include "../../include/bp_diff.h"
 
 using namespace bpk;
+
+/* Slots per lane of the adjoint for four or fewer states (one above). */
+#ifndef BP_ADJ_SLOTS_SMALL
+#define BP_ADJ_SLOTS_SMALL 2
+#endif
 
 namespace {
 
@@ -82,7 +87,7 @@ __global__ void k_copy(const float *__restrict__ src, float *__restrict__ dst, s
  * configurations idx in which the parent is in state j, in ascending idx
  * order -- the blocks of the reader's loop, in its order -- written to the
  * forward slot and, transposed, to the reverse slot.  Loads are issued eight
- * at a time ahead of the adds, which stay in order. */
+ * at a time ahead of the adds, which stay in order; no division per entry. */
 __global__ void k_tables(const float *__restrict__ P, const ParentRec *__restrict__ parents,
                          const unsigned *__restrict__ par_def, const unsigned *__restrict__ def_tb,
                          const unsigned *__restrict__ def_slice, float *__restrict__ J, unsigned nrec, unsigned S)
@@ -100,12 +105,18 @@ __global__ void k_tables(const float *__restrict__ P, const ParentRec *__restric
     const float *T = P + def_tb[d] + (size_t)k * slice + (size_t)j * o;
     unsigned cnt = slice / n; /* entries with the parent in state j */
     float acc = 0.0f;
+    /* entry m of the sum is T[(m / o) * n * o + m % o]; (hi, lo) = (m / o,
+     * m % o) is carried along instead of divided out */
+    unsigned hi = 0, lo = 0;
     for (unsigned m0 = 0; m0 < cnt; m0 += 8) {
         float v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            unsigned m = m0 + u;
-            v[u] = (m < cnt) ? T[(size_t)(m / o) * n * o + (m % o)] : 0.0f;
+            v[u] = (m0 + u < cnt) ? T[(size_t)hi * n * o + lo] : 0.0f;
+            if (++lo == o) {
+                lo = 0;
+                ++hi;
+            }
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u)
@@ -153,11 +164,10 @@ __device__ __forceinline__ void sum_out_rows(const float *__restrict__ rows, con
     }
 }
 
-template <int S, int STEP0>
+template <int S, int STEP0, int R>
 __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
 {
     constexpr int SP = Stride<S>::value;
-    constexpr int R = SlotsPerLane<S>::value;
     constexpr int W = WAVE * R;
     __shared__ float lm[WAVES_PER_BLOCK][S * W];
     __shared__ float ldt[WAVES_PER_BLOCK][S * W];
@@ -389,24 +399,30 @@ __global__ void k_parent_grad(const float *__restrict__ jbar, const ParentRec *_
 }
 
 /* dL/dparam: the transpose of k_tables and k_initial, from the per-parent
- * sums G.  Parent records and G entries of up to eight parents are loaded
- * before any is added. */
+ * sums G: parameter k * slice + idx of a definition gets, for each parent
+ * record in order, G[record][k][j] with j that parent's state in
+ * configuration idx.  Every node has S states, so a definition's records,
+ * which run from its last GIVEN (offset 1) to its first, read the base-S
+ * digits of idx from the lowest up: one division by the constant S per
+ * record.  The G entries of up to eight records are loaded before any is
+ * added. */
 struct DefInfo {
     unsigned tb, slice, pbegin, pend, prior, pad0, pad1, pad2;
 };
 
+template <int S>
 __global__ void k_param_grad(const float *__restrict__ G, const float *__restrict__ dst0,
                              const unsigned *__restrict__ def_of, const DefInfo *__restrict__ defs,
-                             const ParentRec *__restrict__ parents, float *__restrict__ out, unsigned num_params,
-                             unsigned S, unsigned SP)
+                             float *__restrict__ out, unsigned num_params, unsigned SP)
 {
+    constexpr unsigned SS = (unsigned)S * S;
     unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= num_params)
         return;
     DefInfo di = defs[def_of[p]];
     unsigned local = p - di.tb;
-    unsigned k = local / di.slice, idx = local % di.slice;
-    size_t SS = (size_t)S * S;
+    unsigned k = local / di.slice, x = local - k * di.slice;
+    const float *Gk = G + k * S;
     float g = 0.0f;
     for (unsigned base = di.pbegin; base < di.pend; base += 8) {
         float f[8];
@@ -414,9 +430,9 @@ __global__ void k_param_grad(const float *__restrict__ G, const float *__restric
         for (int u = 0; u < 8; ++u) {
             f[u] = 0.0f;
             if (base + u < di.pend) {
-                ParentRec pr = parents[base + u];
-                unsigned j = (idx / pr.offset) % pr.n_par;
-                f[u] = G[(size_t)(base + u) * SS + k * S + j];
+                unsigned q = x / (unsigned)S, j = x - q * (unsigned)S;
+                x = q;
+                f[u] = Gk[(size_t)(base + u) * SS + j];
             }
         }
 #pragma unroll
@@ -519,6 +535,9 @@ struct bp_diff {
     float *w[2] = {nullptr, nullptr};
     float *jbar = nullptr;
     float *dst0p = nullptr, *dst0 = nullptr;
+    /* the adjoint's chunks: at most 64 x adj_R slots and nodes each */
+    uint2 *adj_chunk = nullptr;
+    unsigned adj_R = 1, adj_nchunks = 0, adj_G = 8;
 
     /* iterates: checkpoints t = 0, c, 2c, ...; a segment's recomputed ones;
      * the forward's scratch; the last beliefs */
@@ -558,7 +577,7 @@ extern "C" void bp_diff_free(bp_diff *x)
         (void)hipDeviceSynchronize();
     }
     diff_release_storage(x);
-    void *ps[] = {x->defs, x->G, x->perm_d, x->inv_d, x->row_first_d, x->row_stride_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
+    void *ps[] = {x->adj_chunk, x->defs, x->G, x->perm_d, x->inv_d, x->row_first_d, x->row_stride_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
                   x->def_pbegin, x->def_prior, x->parents, x->P, x->st0, x->gP, x->gin, x->gst[0], x->gst[1],
                   x->w[0], x->w[1], x->jbar, x->dst0p, x->dst0};
     for (void *p : ps)
@@ -583,7 +602,7 @@ static int diff_validate(const bp_bn *bn, unsigned npd, Graph_t g)
 {
     unsigned V = g->current_num_vertices;
     unsigned S = V ? g->node_num_vars[0] : 1;
-    const unsigned R = S <= 4 ? (unsigned)BP_SLOTS_SMALL : 1u;
+    const unsigned R = S <= 4 ? (unsigned)BP_ADJ_SLOTS_SMALL : 1u;
     for (unsigned v = 0; v < V; ++v)
         if (g->node_num_vars[v] != S) {
             bp_set_error("bp_diff_create: node %u has %u states, node 0 has %u; the adjoint needs one state count", v,
@@ -722,6 +741,21 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
         std::vector<unsigned> fill(out_off.begin(), out_off.end() - 1);
         for (unsigned s = 0; s < E; ++s)
             out_slot[fill[slot_src[s]]++] = s;
+    }
+    {
+        std::vector<unsigned> in_off(V + 1, 0);
+        for (unsigned s = 0; s < E; ++s)
+            in_off[d->perm[g->edges_dest_index[d->slot_edge[s]]] + 1]++;
+        for (unsigned v = 0; v < V; ++v)
+            in_off[v + 1] += in_off[v];
+        x->adj_R = (S <= 4) ? (unsigned)BP_ADJ_SLOTS_SMALL : 1u;
+        std::vector<uint2> ch;
+        bp_build_chunks(in_off, V, WAVE * x->adj_R, ch);
+        x->adj_nchunks = (unsigned)ch.size() - 1;
+        unsigned want = (x->adj_nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+        x->adj_G = std::max(8u, (want + 7u) & ~7u);
+        if (upload_vec(&x->adj_chunk, ch))
+            return -1;
     }
     size_t VS = (size_t)V * d->SP, ES = (size_t)E * d->SP;
     if (upload_vec(&x->perm_d, d->perm) || upload_vec(&x->inv_d, d->inv) || upload_vec(&x->prior_tb, prior_tb) ||
@@ -986,8 +1020,8 @@ static void adj_step(bp_diff *x, int step0, const float *st_prev, const float *s
     a.out_off = x->out_off;
     a.out_slot = x->out_slot;
     a.J = d->J;
-    a.chunk = d->chunk;
-    a.nchunks = d->nchunks;
+    a.chunk = x->adj_chunk;
+    a.nchunks = x->adj_nchunks;
     a.st_prev = st_prev;
     a.st_cur = st_cur;
     a.bsrc = bsrc;
@@ -997,13 +1031,14 @@ static void adj_step(bp_diff *x, int step0, const float *st_prev, const float *s
     a.w_out = w_out;
     a.jbar = x->jbar;
     a.jbar_first = first;
-    dim3 grid(d->G), block(BLOCK);
+    dim3 grid(x->adj_G), block(BLOCK);
+    constexpr int RS = BP_ADJ_SLOTS_SMALL;
 #define BP_ADJ_CASE(SS)                                                                   \
     case SS:                                                                              \
         if (step0)                                                                        \
-            hipLaunchKernelGGL((k_adjoint<SS, 1>), grid, block, 0, st, a);                \
+            hipLaunchKernelGGL((k_adjoint<SS, 1, (SS <= 4 ? RS : 1)>), grid, block, 0, st, a); \
         else                                                                              \
-            hipLaunchKernelGGL((k_adjoint<SS, 0>), grid, block, 0, st, a);                \
+            hipLaunchKernelGGL((k_adjoint<SS, 0, (SS <= 4 ? RS : 1)>), grid, block, 0, st, a); \
         break;
     switch (d->S) {
         BP_ADJ_CASE(1)
@@ -1076,8 +1111,24 @@ static void enqueue_backward(bp_diff *x, hipStream_t st)
         hipLaunchKernelGGL(k_parent_grad, dim3(blocks_for(nG)), dim3(256), 0, st, x->jbar, x->parents, x->G, x->nrec,
                            x->S);
     if (x->num_params)
-        hipLaunchKernelGGL(k_param_grad, dim3(blocks_for(x->num_params)), dim3(256), 0, st, x->G, x->dst0, x->def_of,
-                           x->defs, x->parents, x->gP, x->num_params, x->S, x->SP);
+        switch (x->S) {
+#define BP_PG_CASE(SS)                                                                                        \
+    case SS:                                                                                                  \
+        hipLaunchKernelGGL(k_param_grad<SS>, dim3(blocks_for(x->num_params)), dim3(256), 0, st, x->G, x->dst0, \
+                           x->def_of, x->defs, x->gP, x->num_params, x->SP);                                   \
+        break;
+            BP_PG_CASE(1)
+            BP_PG_CASE(2)
+            BP_PG_CASE(3)
+            BP_PG_CASE(4)
+            BP_PG_CASE(5)
+            BP_PG_CASE(6)
+            BP_PG_CASE(7)
+            BP_PG_CASE(8)
+            BP_PG_CASE(9)
+            BP_PG_CASE(10)
+#undef BP_PG_CASE
+        }
 }
 
 extern "C" int bp_diff_backward_ex(bp_diff *x, const float *grad_beliefs, float *grad_params, float *grad_tables,
