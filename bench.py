@@ -97,8 +97,9 @@ def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: i
     in-degree 0-4, states 2-8, 8 evidence sets of 3% each, 30 iterations):
     one batch (hipGraph-captured launches, tuned shape) against the plain
     path run network after network (init + K iterations, each finished
-    before the next starts).  plain_leg=False skips the plain comparison
-    (for profiling the batch kernels alone)."""
+    before the next starts; and the same launches with no wait between
+    networks).  plain_leg=False skips the plain comparison (for profiling
+    the batch kernels alone)."""
     from workloads import batched_networks
 
     lib = B.load()
@@ -139,13 +140,27 @@ def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: i
         d.iterate(K)
         d.sync()
     t_plain = time.perf_counter() - t0
+    # the same launches with no wait between networks (each handle has its
+    # own stream, so they may overlap)
+    t0 = time.perf_counter()
+    for d in plain:
+        d.init()
+        d.iterate(K)
+    for d in plain:
+        d.sync()
+    t_queued = time.perf_counter() - t0
     out = {"workload": f"{count} networks x {evidence_sets} evidence sets: {info['V']} nodes, {info['E']} message "
                        f"edges, {K} iterations", "instances": int(info["instances"]),
            "batched_ms": t_batch * 1e3, "plain_ms": t_plain * 1e3 if plain_leg else None,
            "speedup": t_plain / t_batch if plain_leg else None,
+           "plain_queued_ms": t_queued * 1e3 if plain_leg else None,
+           "speedup_vs_queued": t_queued / t_batch if plain_leg else None,
            "shape": {"slots_per_lane": shape[0], "lanes_per_slot": shape[1], "queue_block": shape[2]},
            "iteration_kernel_ms": kern_ms,
            "iteration_achieved_GBs": info["alg_bytes"] / (kern_ms * 1e-3) / 1e9,
+           "iteration_frac": info["alg_bytes"] / (kern_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "iteration_algorithmic_bytes": info["alg_bytes"],
+           "iteration_traffic_bytes": read_traffic("batch_iter_c4_traffic.txt") if count == 512 else None,
            "host_build_s": build_s,
            "msg_updates_per_s": info["E"] * K / t_batch}
     for d in plain:
