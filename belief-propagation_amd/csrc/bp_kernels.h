@@ -60,8 +60,18 @@
 
 namespace bpk {
 
+#ifndef BP_BLOCK
+#define BP_BLOCK 256
+#endif
+/* Lanes-per-slot path: 1 writes each new message as one whole row from its
+ * LDS copy, one lane per slot, after the passes; 0 lets every lane store
+ * the entries it formed. */
+#ifndef BP_ROWS_MSG_FROM_LDS
+#define BP_ROWS_MSG_FROM_LDS 1
+#endif
+
 constexpr int WAVE = 64;
-constexpr int BLOCK = 256;
+constexpr int BLOCK = BP_BLOCK;
 constexpr int WAVES_PER_BLOCK = BLOCK / WAVE;
 
 /* ITER  one node-centric loopy iteration (graph.c:1121-1183)
@@ -427,23 +437,20 @@ __device__ __forceinline__ void write_node(const LoopyArgs &a, unsigned v, const
         for (int k = 0; k < SP; ++k)
             a.states_out[(size_t)v * SP + k] = (k < S && (!MIXED || (unsigned)k < n)) ? st[k] : 0.0f;
     }
+    /* whole padded rows (zeros beyond the node's state count): a partly
+     * written row costs more than the zeros (measured on mixed networks) */
     if (MODE == ITER) {
-        if (MIXED) {
+        float row[SP];
 #pragma unroll
-            for (int k = 0; k < S; ++k)
-                if ((unsigned)k < n)
-                    a.states[(size_t)v * SP + k] = st[k];
-        } else {
-            float row[SP];
-#pragma unroll
-            for (int k = 0; k < SP; ++k)
-                row[k] = (k < S) ? st[k] : 0.0f;
-            store_row_nt<SP>(a.states + (size_t)v * SP, row);
-        }
+        for (int k = 0; k < SP; ++k)
+            row[k] = (k < S && (!MIXED || (unsigned)k < n)) ? st[k] : 0.0f;
+        store_row_nt<SP>(a.states + (size_t)v * SP, row);
     }
+    float row[SP];
 #pragma unroll
     for (int k = 0; k < SP; ++k)
-        a.buf_out[(size_t)v * SP + k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
+        row[k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
+    store_row_nt<SP>(a.buf_out + (size_t)v * SP, row);
 }
 
 /* Row loads: 8- and 16-byte vector loads where the row size allows. */
@@ -812,7 +819,7 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                     }
                     lds[i * W + q] = m[r];
                 }
-                if (MODE == ITER || MODE == INIT || MODE == EDGE)
+                if ((MODE == ITER || MODE == INIT || MODE == EDGE) && !BP_ROWS_MSG_FROM_LDS)
                     a.msg_out[(size_t)s * SP + i] = (i < (unsigned)S) ? m[r] : 0.0f;
             }
         }
@@ -824,6 +831,18 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
         run_pass(p, tv);
     }
     wave_sync();
+    if ((MODE == ITER || MODE == INIT || MODE == EDGE) && BP_ROWS_MSG_FROM_LDS) {
+        /* the new messages, one whole row per slot (zeros beyond the
+         * message's entries are in LDS already) */
+        unsigned s = sb + lane;
+        if (s < se) {
+            float row[SP];
+#pragma unroll
+            for (int i = 0; i < SP; ++i)
+                row[i] = (i < S) ? lds[i * W + lane] : 0.0f;
+            store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
+        }
+    }
     if (v < ne)
         node_update<S, MIXED, MODE>(a, v, q0, q1, st, n, lds, W);
 }

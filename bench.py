@@ -47,6 +47,21 @@ def read_traffic(name: str):
     return None
 
 
+def progress(msg: str) -> None:
+    """A progress line on stderr (stdout carries only the result line)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_threads() -> int:
     try:
         return len(os.sched_getaffinity(0))
@@ -111,8 +126,11 @@ def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: i
         if plain_leg:
             plain.append(B.Device(g, 0))
         lib.graph_destroy(g)
+        if e == evidence_sets - 1 and (i + 1) % 64 == 0:
+            progress(f"batched form: {i + 1} of {count} networks built ({time.perf_counter() - t0:.0f} s)")
     build_s = time.perf_counter() - t0
     batch.upload(0)
+    progress("batched form: uploaded; tuning")
     batch.tune(3, str(ROOT / "profiles" / "batch_shapes.json"))
     shape = batch.config()
 
@@ -130,6 +148,7 @@ def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: i
     batch.init()
     kern_ms = batch.profile(5)
     info = batch.info()
+    progress("batched form: batch timed; plain per-network runs")
     for d in plain[:32]:
         d.init()
         d.iterate(K)
@@ -256,21 +275,36 @@ def main():
         import oracle_ctypes as O
 
         threads = cpu_threads()
-        h = B.bn_to_graph(bn)
-        t0 = time.perf_counter()
-        O.init_previous_edge(h)
-        for _ in range(ITERATIONS):
-            O.loopy_iteration(h, threads=threads)
-        t_cpu = time.perf_counter() - t0
-        parity = bool(B.states(h).tobytes() == gpu_states.tobytes())
+
+        def cpu_run(nthreads):
+            h = B.bn_to_graph(bn)
+            t0 = time.perf_counter()
+            O.init_previous_edge(h)
+            t1 = time.perf_counter()
+            for _ in range(ITERATIONS):
+                O.loopy_iteration(h, threads=nthreads)
+            t2 = time.perf_counter()
+            st = B.states(h).copy()
+            lib.graph_destroy(h)
+            return t2 - t0, t2 - t1, st
+
+        progress(f"CPU baseline: {threads} threads, then 1")
+        t_cpu, t_iter, cpu_states = cpu_run(threads)
+        parity = bool(cpu_states.tobytes() == gpu_states.tobytes())
+        t_ser, t_ser_iter, ser_states = cpu_run(1)
+        parity = parity and bool(ser_states.tobytes() == gpu_states.tobytes())
         cpu = {"value": E * ITERATIONS / t_cpu, "unit": "msg-updates/s", "cores": threads, "kind": "port",
                "sample": f"the full workload once: init + {ITERATIONS} iterations of the oracle's OpenMP node loop "
-                         f"({t_cpu:.2f} s wall, {threads} threads)"}
-        lib.graph_destroy(h)
+                         f"({t_cpu:.2f} s wall, {threads} threads, not pinned)",
+               "iterations_only_value": E * ITERATIONS / t_iter,
+               "serial_value": E * ITERATIONS / t_ser,
+               "serial_iterations_only_value": E * ITERATIONS / t_ser_iter,
+               "nproc": os.cpu_count(), "cpu_model": cpu_model()}
 
     forms = {}
     wanted = [f for f in args.forms.split(",") if f]
     if "adjoint" in wanted:
+        progress("adjoint form")
         forms["adjoint"] = bench_adjoint()
     if "batched" in wanted:
         forms["batched"] = bench_batched(count=args.batch_nets, plain_leg=not args.no_batch_plain)
