@@ -36,7 +36,10 @@ struct bp_dev {
     unsigned *in_off = nullptr;       /* [V+1] slot range per node */
     unsigned *slot_src = nullptr;     /* [E] */
     unsigned char *slot_xy = nullptr; /* [E] */
-    float *J = nullptr;               /* [E][S*S] */
+    float *J = nullptr;               /* [E][S*S]; mixed state counts: each slot's x*y floats, packed */
+    unsigned *slot_toff = nullptr;    /* mixed state counts: [E] offset of each slot's table in J */
+    std::vector<unsigned> toff;       /* the same on the host */
+    size_t jfloats = 0;               /* floats in J */
     unsigned char *nvars = nullptr;   /* [V] */
     float *states = nullptr;          /* [V][SP] */
     float *buf[2] = {nullptr, nullptr};

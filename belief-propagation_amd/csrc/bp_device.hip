@@ -105,7 +105,7 @@ __global__ void k_send_one(const float *J, const float *b, unsigned x, unsigned 
 
 static void free_dev_arrays(bp_dev *d)
 {
-    void *ptrs[] = {d->in_off, d->slot_src, d->slot_xy, d->J, d->nvars, d->states, d->buf[0], d->buf[1],
+    void *ptrs[] = {d->slot_toff, d->in_off, d->slot_src, d->slot_xy, d->J, d->nvars, d->states, d->buf[0], d->buf[1],
                     d->msg[0], d->msg[1], d->states0, d->chunk, d->partials, d->deltas, d->last, d->stop};
     for (void *p : ptrs)
         if (p)
@@ -354,14 +354,17 @@ static int upload_state(bp_dev *d, Graph_t g, bool tables)
     d->cur_is_a = (g->current_edge_messages == &g->edges_messages);
     if (tables) {
         unsigned S = d->S;
-        tmp.assign((size_t)d->E * S * S, 0.0f);
+        tmp.assign(d->jfloats, 0.0f);
         for (unsigned s = 0; s < d->E; ++s) {
             unsigned e = d->slot_edge[s];
             unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
             const float *J = g->edges_joint_probabilities + (size_t)e * MAX_STATES * MAX_STATES;
+            /* mixed: x rows of y entries at toff[s]; uniform: [s][S*S] */
+            size_t base = d->mixed ? d->toff[s] : (size_t)s * S * S;
+            unsigned stride = d->mixed ? y : S;
             for (unsigned i = 0; i < x; ++i)
                 for (unsigned j = 0; j < y; ++j)
-                    tmp[(size_t)s * S * S + i * S + j] = J[i * MAX_STATES + j];
+                    tmp[base + i * stride + j] = J[i * MAX_STATES + j];
         }
         HIP_TRY(hipMemcpyAsync(d->J, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
         HIP_TRY(hipStreamSynchronize(d->stream));
@@ -508,8 +511,33 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
     d->alg_bytes = algorithmic_bytes(g);
 
     size_t SP = d->SP;
+    /* mixed state counts: tables packed at their own sizes (a 2x2 table
+     * takes 4 floats, not S*S) behind per-slot offsets */
+    if (d->mixed) {
+        d->toff.resize(E);
+        size_t tot = 0;
+        for (unsigned s = 0; s < E; ++s) {
+            unsigned e = d->slot_edge[s];
+            d->toff[s] = (unsigned)tot;
+            tot += (size_t)g->edges_x_dim[e] * g->edges_y_dim[e];
+        }
+        if (tot >= 0xffffffffull) {
+            bp_set_error("upload: more than 2^32 table entries");
+            bp_free(d);
+            return -1;
+        }
+        d->jfloats = tot;
+        if (dmalloc(&d->slot_toff, E) ||
+            (E && hipMemcpy(d->slot_toff, d->toff.data(), (size_t)E * 4, hipMemcpyHostToDevice) != hipSuccess)) {
+            bp_set_error("upload of the table offsets failed");
+            bp_free(d);
+            return -1;
+        }
+    } else {
+        d->jfloats = (size_t)E * S * S;
+    }
     if (dmalloc(&d->in_off, V + 1) || dmalloc(&d->slot_src, E) || dmalloc(&d->slot_xy, E) ||
-        dmalloc(&d->J, (size_t)E * S * S) || dmalloc(&d->nvars, V) || dmalloc(&d->states, V * SP) ||
+        dmalloc(&d->J, d->jfloats) || dmalloc(&d->nvars, V) || dmalloc(&d->states, V * SP) ||
         dmalloc(&d->buf[0], V * SP) || dmalloc(&d->buf[1], V * SP) || dmalloc(&d->msg[0], (size_t)E * SP) ||
         dmalloc(&d->msg[1], (size_t)E * SP) || dmalloc(&d->states0, V * SP) ||
         dmalloc(&d->chunk, chunks.size()) || dmalloc(&d->partials, (size_t)d->part_cap * d->NP) ||
@@ -581,7 +609,7 @@ LoopyArgs bp_make_args(bp_dev *d)
     a.nchunks = d->nchunks;
     a.partials = d->partials;
     a.stop = nullptr;
-    a.slot_tab = nullptr;
+    a.slot_tab = d->mixed ? d->slot_toff : nullptr;
     a.tabs = nullptr;
     a.chunk_partials = nullptr;
     a.queue = nullptr;

@@ -103,10 +103,12 @@ struct LoopyArgs {
     unsigned nchunks;
     double *partials;            /* [G * waves per block] this launch's per-wave delta partials (ITER) */
     const unsigned *stop;        /* non-null: skip the launch once *stop != 0 */
-    /* pooled tables (batched networks): slot s's table starts at
+    /* pooled tables (batched networks, PACKED): slot s's table starts at
      * tabs + slot_tab[s], row-major with the slot's x rows of y entries,
      * each row padded with zeros to a multiple of four floats (16-byte
-     * aligned rows) */
+     * aligned rows).  Otherwise, with mixed state counts, slot s's table is
+     * J + slot_tab[s], x rows of y entries with no padding (uniform state
+     * counts: J[s][S*S], slot_tab unused). */
     const unsigned *slot_tab;
     const float *tabs;
     double *chunk_partials;      /* non-null: one delta partial per chunk instead of per block */
@@ -245,11 +247,19 @@ __device__ __forceinline__ void load_pooled_row(const float *__restrict__ T, uns
 }
 
 /* The table of slot s as J[i*S + j] (zero beyond x rows and y columns). */
-template <int S, bool PACKED>
+template <int S, bool MIXED, bool PACKED>
 __device__ __forceinline__ void load_table(const LoopyArgs &a, unsigned s, unsigned xy, float *J)
 {
-    if constexpr (!PACKED) {
+    if constexpr (!PACKED && !MIXED) {
         load_row_nt<S * S>(a.J + (size_t)s * S * S, J);
+    } else if constexpr (!PACKED) {
+        unsigned x = xy & 15u, y = xy >> 4;
+        const float *T = a.J + a.slot_tab[s];
+#pragma unroll
+        for (int i = 0; i < S; ++i)
+#pragma unroll
+            for (int j = 0; j < S; ++j)
+                J[i * S + j] = ((unsigned)i < x && (unsigned)j < y) ? T[i * y + j] : 0.0f;
     } else {
         unsigned x = xy & 15u, y = xy >> 4;
         const float *T = a.tabs + a.slot_tab[s];
@@ -271,7 +281,7 @@ template <int S, bool MIXED, bool PACKED>
 __device__ __forceinline__ void send_from_memory(const LoopyArgs &a, unsigned s, unsigned xy, const float *b, float *m)
 {
     unsigned x = MIXED ? (xy & 15u) : (unsigned)S, y = MIXED ? (xy >> 4) : (unsigned)S;
-    const float *T = PACKED ? a.tabs + a.slot_tab[s] : a.J + (size_t)s * S * S;
+    const float *T = PACKED ? a.tabs + a.slot_tab[s] : MIXED ? a.J + a.slot_tab[s] : a.J + (size_t)s * S * S;
     float sum = 0.0f;
 #pragma unroll
     for (int i = 0; i < S; ++i) {
@@ -280,6 +290,10 @@ __device__ __forceinline__ void send_from_memory(const LoopyArgs &a, unsigned s,
             float row[S];
             if constexpr (PACKED) {
                 load_pooled_row<S>(T, i, y, row);
+            } else if constexpr (MIXED) {
+#pragma unroll
+                for (int j = 0; j < S; ++j)
+                    row[j] = (unsigned)j < y ? T[(size_t)i * y + j] : 0.0f;
             } else {
 #pragma unroll
                 for (int j = 0; j < S; ++j)
@@ -579,7 +593,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 u[r] = BP_NT_LOADS ? __builtin_nontemporal_load(a.slot_src + s) : a.slot_src[s];
                 xy[r] = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
                 if constexpr (HOLD)
-                    load_table<S, PACKED>(a, s, xy[r], J[r]);
+                    load_table<S, MIXED, PACKED>(a, s, xy[r], J[r]);
             }
             if (MODE == ITER || MODE == EDGE || MODE == PREP)
                 load_rowp_nt<S, SP>(a.msg_in + (size_t)s * SP, old[r]);
@@ -677,7 +691,7 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
         unsigned s = sb + lane;
         if (s < se) {
             unsigned xy = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
-            unsigned t = PACKED ? a.slot_tab[s] : 0u;
+            unsigned t = (PACKED || MIXED) ? a.slot_tab[s] : 0u;
             float b[S], old[S];
             if (MODE == PREP) {
                 load_rowp<S, SP>(a.msg_in + (size_t)s * SP, b);
@@ -729,13 +743,17 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
             x = xy & 15u;
             y = xy >> 4;
         }
-        const float *T = PACKED ? a.tabs + stage_u[W + q] : a.J + (size_t)s * S * S;
+        const float *T = PACKED ? a.tabs + stage_u[W + q] : MIXED ? a.J + stage_u[W + q] : a.J + (size_t)s * S * S;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             unsigned i = k + r * L;
             if (i < (unsigned)S && (!MIXED || i < x)) {
                 if constexpr (PACKED) {
                     load_pooled_row<S>(T, i, y, tv[r]);
+                } else if constexpr (MIXED) {
+#pragma unroll
+                    for (int j = 0; j < S; ++j)
+                        tv[r][j] = (unsigned)j < y ? T[(size_t)i * y + j] : 0.0f;
                 } else {
 #pragma unroll
                     for (int j = 0; j < S; ++j)

@@ -41,6 +41,7 @@ struct TreeArgs {
     const unsigned *in_off;
     const unsigned char *slot_xy;
     const float *J;
+    const unsigned *slot_tab; /* mixed state counts: table offsets in J (x rows of y, no padding) */
     const unsigned char *nvars;
     const float *states;
     float *msg;
@@ -83,7 +84,18 @@ __global__ __launch_bounds__(256) void k_tree_wave(TreeArgs a, unsigned begin, u
             x = xy & 15u;
             y = xy >> 4;
         }
-        send<S, MIXED>(a.J + (size_t)s * S * S, buf, x, y, m);
+        if constexpr (MIXED) {
+            float Jd[S * S];
+            const float *T = a.J + a.slot_tab[s];
+#pragma unroll
+            for (int i = 0; i < S; ++i)
+#pragma unroll
+                for (int j = 0; j < S; ++j)
+                    Jd[i * S + j] = ((unsigned)i < x && (unsigned)j < y) ? T[i * y + j] : 0.0f;
+            send<S, MIXED>(Jd, buf, x, y, m);
+        } else {
+            send<S, MIXED>(a.J + (size_t)s * S * S, buf, x, y, m);
+        }
 #pragma unroll
         for (int k = 0; k < SP; ++k)
             a.msg[(size_t)s * SP + k] = (k < S) ? m[k] : 0.0f;
@@ -320,6 +332,7 @@ static TreeArgs tree_args(bp_dev *d, const bp_tree_plan *p)
     a.in_off = d->in_off;
     a.slot_xy = d->slot_xy;
     a.J = d->J;
+    a.slot_tab = d->mixed ? d->slot_toff : nullptr;
     a.nvars = d->nvars;
     a.states = d->states;
     a.msg = d->msg[a_index(d)];
