@@ -49,9 +49,10 @@ int bp_set_abort_on_error(int enabled); /* returns the previous setting */
 /* Number of HIP devices visible (0 without a GPU); never fails. */
 int bp_device_count(void);
 
-/* Node numbering on the device for handles uploaded from now on: 1 (the
- * default) renumbers nodes in Cuthill-McKee order for locality of
- * the gathers, 0 keeps the graph's order.  Results are the same either way
+/* Node numbering on the device for handles uploaded, and batch instances
+ * added (bp_batch_add), from now on: 1 (the default) renumbers nodes in
+ * Cuthill-McKee order for locality of the gathers, 0 keeps the graph's
+ * order.  Results are the same either way
  * (each node's in-edges keep the reference's order); only the FP64 delta's
  * summation order, and so its last bits, follow the numbering.  Returns the
  * previous setting. */
