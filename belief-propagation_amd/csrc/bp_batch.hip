@@ -2,14 +2,17 @@
  * bp_batch.hip -- many networks per launch.
  *
  * Layout: the batch is the disjoint union of its instances.  Nodes and slots
- * are numbered instance after instance; within an instance, slots follow
- * the reference's destination-grouped edge order (src/graph/graph.c:285-313)
- * as in bp_upload.  Tables live in one pool of distinct tables
- * (row-major x by y floats); a slot points at its table, or at a stored
- * transpose of it, so every arc's table is stored once for the forward edge,
- * the reverse edge and every evidence set.  Chunks never span two instances,
- * so each chunk's FP64 delta partial belongs to one instance and k_batch_finish
- * sums them per instance in chunk order (deterministic, no atomics).
+ * are numbered instance after instance; within an instance, nodes are in
+ * the device order (Cuthill-McKee, shared by a run of instances over the
+ * same node set) and each node's slots follow the reference's
+ * destination-grouped edge order (src/graph/graph.c:285-313) as in
+ * bp_upload.  Tables live in one pool deduplicated by content (the same arc
+ * under every evidence set), each in the orientation its slots read it, rows
+ * padded to four floats.  Chunks never span two instances, so each chunk's
+ * FP64 delta partial belongs to one instance and k_batch_finish sums them
+ * per instance (deterministic, no atomics).  Chunks run in an order that
+ * merges the instances of a network by node range, so a network's evidence
+ * sets read their shared tables close together in time.
  *
  * The iteration kernel is k_loopy with packed tables and mixed state counts;
  * its arithmetic per node and slot is the plain path's, so every instance

@@ -9,7 +9,9 @@
  *            reference multiplies them;
  *   in_off[V+1]      slot range of each node
  *   slot_src[E]      source node of the slot's edge
- *   J[E][S*S]        the edge's table, J[i][j], i = source state, j = destination state
+ *   J[E][S*S]        the edge's table, J[i][j], i = source state (mixed
+ *                    state counts: x rows of y floats at J + slot_tab[s];
+ *                    batches: a pooled table at tabs + slot_tab[s]), j = destination state
  *   msg[2][E][SP]    messages, double-buffered (old / new)
  *   states[V][SP], buf[2][V][SP]  beliefs and the combine buffer
  *                    states (.) prod(in-messages), double-buffered
