@@ -40,6 +40,8 @@ struct bp_dev {
     unsigned *slot_toff = nullptr;    /* mixed state counts: [E] offset of each slot's table in J */
     std::vector<unsigned> toff;       /* the same on the host */
     size_t jfloats = 0;               /* floats in J */
+    float *h_stage = nullptr;         /* pinned host staging for uploads and downloads */
+    size_t h_stage_n = 0;             /* its capacity in floats */
     unsigned char *nvars = nullptr;   /* [V] */
     float *states = nullptr;          /* [V][SP] */
     float *buf[2] = {nullptr, nullptr};

@@ -20,6 +20,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "bp_dev.h"
@@ -125,6 +126,9 @@ extern "C" void bp_free(bp_dev *d)
         (void)hipGraphExecDestroy(kv.second);
     bp_tree_plan_free(d->tree);
     free_dev_arrays(d);
+    if (d->h_stage)
+        (void)hipHostFree(d->h_stage);
+    d->h_stage = nullptr;
     if (d->stream)
         (void)hipStreamDestroy(d->stream);
     (void)hipSetDevice(prev);
@@ -320,53 +324,121 @@ static int check_graph(Graph_t g)
     return 0;
 }
 
-/* host rows (MAX_STATES stride, first n valid) -> device rows (SP stride, zero padded) */
-static void pack_rows(const float *src, const unsigned *valid, size_t count, unsigned SP, std::vector<float> &out,
-                      const unsigned *order = nullptr)
+/* f(begin, end) over [0, n) split across up to 16 host threads (the
+ * graph's host arrays are MAX_STATES-strided: the repacking is bound by
+ * memory latency, not arithmetic) */
+template <typename F> static void par_for(size_t n, F f)
 {
-    out.assign(count * SP, 0.0f);
-    for (size_t i = 0; i < count; ++i) {
-        size_t from = order ? order[i] : i;
-        unsigned n = valid[i];
-        for (unsigned k = 0; k < n && k < SP; ++k)
-            out[i * SP + k] = src[from * MAX_STATES + k];
+    unsigned T = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < (1u << 16) || T == 1) {
+        f((size_t)0, n);
+        return;
     }
+    size_t step = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t) {
+        size_t a = t * step, b = std::min(n, a + step);
+        if (a < b)
+            th.emplace_back(f, a, b);
+    }
+    for (auto &x : th)
+        x.join();
+}
+
+/* the handle's pinned staging buffer, at least n floats */
+static float *stage_buf(bp_dev *d, size_t n)
+{
+    if (n > d->h_stage_n) {
+        if (d->h_stage)
+            (void)hipHostFree(d->h_stage);
+        d->h_stage = nullptr;
+        d->h_stage_n = 0;
+        if (hipHostMalloc((void **)&d->h_stage, std::max<size_t>(n, 1) * sizeof(float), hipHostMallocDefault) !=
+            hipSuccess) {
+            d->h_stage = nullptr;
+            bp_set_error("pinned staging buffer of %zu floats: allocation failed", n);
+            return nullptr;
+        }
+        d->h_stage_n = n;
+    }
+    return d->h_stage;
+}
+
+/* Host rows (MAX_STATES stride, first n valid) -> device rows (SP stride,
+ * zero padded): host row h goes to device row to[h], walked in host order so
+ * the large host array streams. */
+static void pack_rows(const float *src, const unsigned *valid, size_t count, unsigned SP, float *out,
+                      const unsigned *to)
+{
+    par_for(count, [=](size_t a, size_t b) {
+        for (size_t h = a; h < b; ++h) {
+            float *row = out + (size_t)to[h] * SP;
+            unsigned n = valid[h];
+            for (unsigned k = 0; k < SP; ++k)
+                row[k] = (k < n) ? src[h * MAX_STATES + k] : 0.0f;
+        }
+    });
+}
+
+/* device rows -> host rows (first n of each), in host order */
+static void unpack_rows(const float *in, const unsigned *valid, size_t count, unsigned SP, float *dst,
+                        const unsigned *from)
+{
+    par_for(count, [=](size_t a, size_t b) {
+        for (size_t h = a; h < b; ++h) {
+            const float *row = in + (size_t)from[h] * SP;
+            unsigned n = valid[h];
+            for (unsigned k = 0; k < n; ++k)
+                dst[h * MAX_STATES + k] = row[k];
+        }
+    });
 }
 
 static int upload_state(bp_dev *d, Graph_t g, bool tables)
 {
-    std::vector<float> tmp;
+    size_t VS = (size_t)d->V * d->SP, ES = (size_t)d->E * d->SP;
+    float *tmp = stage_buf(d, std::max(std::max(VS, ES), tables ? d->jfloats : (size_t)0));
+    if (!tmp)
+        return -1;
     /* node states, in device order */
-    pack_rows(g->node_states, d->node_n.data(), d->V, d->SP, tmp, d->inv.data());
-    HIP_TRY(hipMemcpyAsync(d->states, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
-    HIP_TRY(hipMemcpyAsync(d->states0, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    std::vector<unsigned> node_n(d->V);
+    for (unsigned v = 0; v < d->V; ++v)
+        node_n[v] = g->node_num_vars[v];
+    pack_rows(g->node_states, node_n.data(), d->V, d->SP, tmp, d->perm.data());
+    HIP_TRY(hipMemcpyAsync(d->states, tmp, VS * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemcpyAsync(d->states0, tmp, VS * sizeof(float), hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
     /* messages in slot order: msg[0] = previous, msg[1] = current */
-    std::vector<unsigned> slot_x(d->E);
-    for (unsigned s = 0; s < d->E; ++s)
-        slot_x[s] = d->edge_x[d->slot_edge[s]];
-    pack_rows(*g->previous_edge_messages, slot_x.data(), d->E, d->SP, tmp, d->slot_edge.data());
-    HIP_TRY(hipMemcpyAsync(d->msg[0], tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    pack_rows(*g->previous_edge_messages, d->edge_x.data(), d->E, d->SP, tmp, d->edge_slot.data());
+    HIP_TRY(hipMemcpyAsync(d->msg[0], tmp, ES * sizeof(float), hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
-    pack_rows(*g->current_edge_messages, slot_x.data(), d->E, d->SP, tmp, d->slot_edge.data());
-    HIP_TRY(hipMemcpyAsync(d->msg[1], tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+    pack_rows(*g->current_edge_messages, d->edge_x.data(), d->E, d->SP, tmp, d->edge_slot.data());
+    HIP_TRY(hipMemcpyAsync(d->msg[1], tmp, ES * sizeof(float), hipMemcpyHostToDevice, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
     d->cur_is_a = (g->current_edge_messages == &g->edges_messages);
     if (tables) {
         unsigned S = d->S;
-        tmp.assign(d->jfloats, 0.0f);
-        for (unsigned s = 0; s < d->E; ++s) {
-            unsigned e = d->slot_edge[s];
-            unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
-            const float *J = g->edges_joint_probabilities + (size_t)e * MAX_STATES * MAX_STATES;
-            /* mixed: x rows of y entries at toff[s]; uniform: [s][S*S] */
-            size_t base = d->mixed ? d->toff[s] : (size_t)s * S * S;
-            unsigned stride = d->mixed ? y : S;
-            for (unsigned i = 0; i < x; ++i)
-                for (unsigned j = 0; j < y; ++j)
-                    tmp[base + i * stride + j] = J[i * MAX_STATES + j];
-        }
-        HIP_TRY(hipMemcpyAsync(d->J, tmp.data(), tmp.size() * sizeof(float), hipMemcpyHostToDevice, d->stream));
+        /* mixed: x rows of y entries at toff[s]; uniform: [s][S*S], zeros
+         * beyond x rows and y columns */
+        par_for(d->E, [&](size_t a, size_t b) {
+            for (size_t e = a; e < b; ++e) {
+                unsigned s = d->edge_slot[e];
+                unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
+                const float *J = g->edges_joint_probabilities + e * MAX_STATES * MAX_STATES;
+                if (d->mixed) {
+                    float *T = tmp + d->toff[s];
+                    for (unsigned i = 0; i < x; ++i)
+                        for (unsigned j = 0; j < y; ++j)
+                            T[i * y + j] = J[i * MAX_STATES + j];
+                } else {
+                    float *T = tmp + (size_t)s * S * S;
+                    for (unsigned i = 0; i < S; ++i)
+                        for (unsigned j = 0; j < S; ++j)
+                            T[i * S + j] = (i < x && j < y) ? J[i * MAX_STATES + j] : 0.0f;
+                }
+            }
+        });
+        HIP_TRY(hipMemcpyAsync(d->J, tmp, d->jfloats * sizeof(float), hipMemcpyHostToDevice, d->stream));
         HIP_TRY(hipStreamSynchronize(d->stream));
     }
     d->latest = 0;
@@ -1052,15 +1124,13 @@ extern "C" int bp_download_states(bp_dev *d, float *out)
 
 static int download_msgs(bp_dev *d, const float *dev, float *host)
 {
-    std::vector<float> tmp((size_t)d->E * d->SP);
-    HIP_TRY(hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    size_t ES = (size_t)d->E * d->SP;
+    float *tmp = stage_buf(d, ES);
+    if (!tmp)
+        return -1;
+    HIP_TRY(hipMemcpyAsync(tmp, dev, ES * sizeof(float), hipMemcpyDeviceToHost, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
-    for (unsigned s = 0; s < d->E; ++s) {
-        unsigned e = d->slot_edge[s];
-        unsigned x = d->edge_x[e];
-        for (unsigned k = 0; k < x; ++k)
-            host[(size_t)e * MAX_STATES + k] = tmp[(size_t)s * d->SP + k];
-    }
+    unpack_rows(tmp, d->edge_x.data(), d->E, d->SP, host, d->edge_slot.data());
     return 0;
 }
 
@@ -1072,12 +1142,18 @@ extern "C" int bp_download(bp_dev *d, Graph_t g)
         return -1;
     }
     HIP_TRY(hipSetDevice(d->device));
-    std::vector<float> st((size_t)d->V * d->SP);
-    HIP_TRY(hipMemcpyAsync(st.data(), d->states, st.size() * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+    size_t VS = (size_t)d->V * d->SP;
+    float *st = stage_buf(d, VS);
+    if (!st)
+        return -1;
+    HIP_TRY(hipMemcpyAsync(st, d->states, VS * sizeof(float), hipMemcpyDeviceToHost, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
-    for (unsigned i = 0; i < d->V; ++i)
-        for (unsigned k = 0; k < d->node_n[i]; ++k)
-            g->node_states[(size_t)d->inv[i] * MAX_STATES + k] = st[(size_t)i * d->SP + k];
+    {
+        std::vector<unsigned> node_n(d->V);
+        for (unsigned v = 0; v < d->V; ++v)
+            node_n[v] = g->node_num_vars[v];
+        unpack_rows(st, node_n.data(), d->V, d->SP, g->node_states, d->perm.data());
+    }
     /* one pointer swap per iteration run on the device (graph.c:1179-1182) */
     if (d->swaps & 1u) {
         float **t = g->previous_edge_messages;
@@ -1116,23 +1192,25 @@ extern "C" double bp_loopy_algorithmic_bytes(const bp_dev *d) { return d ? d->al
 /* The device mirror of a Graph_t, rebuilt when its structure changed;
  * states, tables and messages are re-read on every call because callers
  * may write the public arrays between calls. */
-extern "C" bp_dev *bp_dropin_device(Graph_t g, const char *who)
+static bp_dev *dropin_device_status(Graph_t g)
 {
     struct bp_graph_impl *impl = bp_impl(g);
     bp_dev *d = (bp_dev *)impl->dev_cache;
-    if (d && impl->dev_cache_generation == impl->generation) {
-        if (bp_upload_state(d, g) == 0)
-            return d;
-        bp_fail("%s: %s", who, bp_last_error());
-        return nullptr;
-    }
+    if (d && impl->dev_cache_generation == impl->generation)
+        return bp_upload_state(d, g) == 0 ? d : nullptr;
     bp_dev_cache_release(impl);
-    if (upload_impl(g, -1, &d, false) != 0) {
-        bp_fail("%s: %s", who, bp_last_error());
+    if (upload_impl(g, -1, &d, false) != 0)
         return nullptr;
-    }
     impl->dev_cache = d;
     impl->dev_cache_generation = impl->generation;
+    return d;
+}
+
+extern "C" bp_dev *bp_dropin_device(Graph_t g, const char *who)
+{
+    bp_dev *d = dropin_device_status(g);
+    if (!d)
+        bp_fail("%s: %s", who, bp_last_error());
     return d;
 }
 
@@ -1214,19 +1292,24 @@ extern "C" void loopy_propagate_edge_one_iteration(Graph_t g)
 }
 
 /* The CUDA driver's contract (belief-propagation.cu:529): the caller has run
- * init_previous_edge; the graph is uploaded, iterated to convergence and
- * downloaded in one call. */
+ * init_previous_edge; the graph's states, tables and messages are uploaded,
+ * iterated to convergence and downloaded in one call.  The device structure
+ * is the graph's cached mirror (built on the first call, rebuilt when the
+ * structure changes through the API), as for the other drop-in entry
+ * points: rebuilding it on every call took about 90% of a call's time on a
+ * 1M-arc network. */
 static unsigned int hip_until(Graph_t g, int mode, float convergence, unsigned int max_iterations)
 {
-    bp_dev *d = nullptr;
     bp_clear_error();
-    if (upload_impl(g, -1, &d, false) != 0)
+    bp_dev *d = dropin_device_status(g);
+    if (!d)
         return 0xffffffffu;
     unsigned int result = 0;
-    int rc = run_until(d, mode, convergence, max_iterations, &result, nullptr);
+    int rc = hipSetDevice(d->device) == hipSuccess ? 0 : -1;
+    if (rc == 0)
+        rc = run_until(d, mode, convergence, max_iterations, &result, nullptr);
     if (rc == 0)
         rc = bp_download(d, g);
-    bp_free(d);
     return rc == 0 ? result : 0xffffffffu;
 }
 

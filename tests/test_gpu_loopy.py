@@ -367,3 +367,30 @@ def test_isolated_nodes_and_chunks_without_slots(lib, orc, edges):
     dev.free()
     lib.graph_destroy(g)
     lib.graph_destroy(h)
+
+
+def test_hip_entry_point_rereads_the_arrays_on_a_second_call(lib, orc):
+    """loopy_propagate_until_hip keeps the graph's device mirror between
+    calls; a table the caller changes between calls is used by the next
+    call, as the oracle doing the same sequence shows."""
+    d = golden("c1")
+    conv, mx = 1e-3, 200
+    g = graph_for_case(d)
+    h = graph_for_case(d)
+    e = 7
+    for rnd in range(2):
+        if rnd == 1:
+            for x in (g, h):
+                J = B.view_f32(x.contents.edges_joint_probabilities, x.contents.current_num_edges * 100)
+                J[e * 100 + 0] *= 0.25
+                J[e * 100 + 11] += 0.5
+        orc.init_previous_edge(h)
+        want = _expected_until(orc, h, orc.loopy_iteration, conv, mx, False)
+        lib.init_previous_edge(g)
+        got = lib.loopy_propagate_until_hip(g, conv, mx)
+        assert got == want, f"round {rnd}"
+        assert same_bits(B.states(g), B.states(h)), f"round {rnd}"
+        assert same_bits(B.messages(g, "cur"), B.messages(h, "cur")), f"round {rnd}"
+        assert same_bits(B.messages(g, "prev"), B.messages(h, "prev")), f"round {rnd}"
+    lib.graph_destroy(g)
+    lib.graph_destroy(h)
