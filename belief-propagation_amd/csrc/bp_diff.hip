@@ -407,7 +407,7 @@ __global__ void k_parent_grad(const float *__restrict__ jbar, const ParentRec *_
  * record.  The G entries of up to eight records are loaded before any is
  * added. */
 struct DefInfo {
-    unsigned tb, slice, pbegin, pend, prior, pad0, pad1, pad2;
+    unsigned tb, slice, pbegin, pend, prior, rbegin, pad1, pad2; /* rbegin: the definition's first softmax row */
 };
 
 template <int S>
@@ -445,6 +445,64 @@ __global__ void k_param_grad(const float *__restrict__ G, const float *__restric
     out[p] = g;
 }
 
+/* dL/dtheta for the log-parameters directly: one thread per softmax row
+ * (a definition's child states under one parent configuration cc).  Every
+ * entry of the row has the same configuration, so its digits are formed
+ * once; each entry's parameter gradient is k_param_grad's sum, in its
+ * order, and the row then goes through the softmax's transpose
+ * dtheta = P (.) (dP - <P, dP>) -- the same operations as k_param_grad
+ * followed by k_softmax_back, without the parameter gradient in memory. */
+template <int S>
+__global__ void k_param_grad_log(const float *__restrict__ G, const float *__restrict__ dst0,
+                                 const unsigned *__restrict__ row_def, const DefInfo *__restrict__ defs,
+                                 const float *__restrict__ P, float *__restrict__ gtheta, unsigned nrows, unsigned SP)
+{
+    constexpr unsigned SS = (unsigned)S * S;
+    constexpr int U = (S <= 4) ? 8 : 4;
+    unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows)
+        return;
+    DefInfo di = defs[row_def[r]];
+    unsigned cc = r - di.rbegin, x = cc;
+    float g[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        g[k] = 0.0f;
+    for (unsigned base = di.pbegin; base < di.pend; base += U) {
+        float f[U][S];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (base + u < di.pend) {
+                unsigned q = x / (unsigned)S, j = x - q * (unsigned)S;
+                x = q;
+                const float *Gr = G + (size_t)(base + u) * SS + j;
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    f[u][k] = Gr[k * S];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (base + u < di.pend)
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    g[k] = g[k] + f[u][k];
+    }
+    if (di.prior != NONE)
+#pragma unroll
+        for (int k = 0; k < S; ++k)
+            g[k] = g[k] + dst0[(size_t)di.prior * SP + k];
+    float pv[S], dot = 0.0f;
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        pv[k] = P[di.tb + (size_t)k * di.slice + cc];
+        dot += pv[k] * g[k];
+    }
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        gtheta[di.tb + (size_t)k * di.slice + cc] = pv[k] * (g[k] - dot);
+}
+
 /* P = softmax of theta over each row (a definition's child states under
  * one parent configuration): entries first[r] + k * stride[r], k < S */
 __global__ void k_softmax(const float *__restrict__ theta, const unsigned *__restrict__ first,
@@ -464,22 +522,6 @@ __global__ void k_softmax(const float *__restrict__ theta, const unsigned *__res
     }
     for (unsigned k = 0; k < S; ++k)
         P[f + k * st] = e[k] / sum;
-}
-
-/* dtheta = P (.) (dP - <P, dP>) per row */
-__global__ void k_softmax_back(const float *__restrict__ P, const float *__restrict__ gP,
-                               const unsigned *__restrict__ first, const unsigned *__restrict__ stride,
-                               float *__restrict__ gtheta, unsigned nrows, unsigned S)
-{
-    unsigned r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows)
-        return;
-    unsigned f = first[r], st = stride[r];
-    float dot = 0.0f;
-    for (unsigned k = 0; k < S; ++k)
-        dot += P[f + k * st] * gP[f + k * st];
-    for (unsigned k = 0; k < S; ++k)
-        gtheta[f + k * st] = P[f + k * st] * (gP[f + k * st] - dot);
 }
 
 /* dst[i] = src[idx[i]] for rows of SP floats (between the caller's node
@@ -528,8 +570,8 @@ struct bp_diff {
     float *P = nullptr;     /* parameters */
     unsigned *perm_d = nullptr, *inv_d = nullptr; /* node -> device node, device node -> node */
     unsigned *row_first_d = nullptr, *row_stride_d = nullptr; /* softmax rows */
+    unsigned *row_def_d = nullptr;                            /* each softmax row's definition */
     float *st0 = nullptr;   /* initial beliefs */
-    float *gP = nullptr;    /* dL/dparams */
     float *gin = nullptr;   /* dL/dbeliefs (copy) */
     float *gst[2] = {nullptr, nullptr};
     float *w[2] = {nullptr, nullptr};
@@ -577,8 +619,8 @@ extern "C" void bp_diff_free(bp_diff *x)
         (void)hipDeviceSynchronize();
     }
     diff_release_storage(x);
-    void *ps[] = {x->adj_chunk, x->defs, x->G, x->perm_d, x->inv_d, x->row_first_d, x->row_stride_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
-                  x->def_pbegin, x->def_prior, x->parents, x->P, x->st0, x->gP, x->gin, x->gst[0], x->gst[1],
+    void *ps[] = {x->adj_chunk, x->defs, x->G, x->perm_d, x->inv_d, x->row_first_d, x->row_stride_d, x->row_def_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
+                  x->def_pbegin, x->def_prior, x->parents, x->P, x->st0, x->gin, x->gst[0], x->gst[1],
                   x->w[0], x->w[1], x->jbar, x->dst0p, x->dst0};
     for (void *p : ps)
         if (p)
@@ -710,22 +752,29 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
     std::vector<DefInfo> definfo(npd);
     for (unsigned dd = 0; dd < npd; ++dd)
         definfo[dd] = DefInfo{def_tb[dd], def_slice[dd], def_pbegin[dd], def_pbegin[dd + 1], def_prior[dd], 0, 0, 0};
+    for (unsigned dd = 0, rows = 0; dd < npd; ++dd) {
+        definfo[dd].rbegin = rows;
+        rows += def_slice[dd];
+    }
     /* softmax rows */
     x->row_first.clear();
     x->row_stride.clear();
     x->row_len.clear();
+    std::vector<unsigned> row_def;
     for (unsigned dd = 0; dd < npd; ++dd) {
         unsigned nch = bn->num_states[bn->def_for[dd]];
         for (unsigned cc = 0; cc < def_slice[dd]; ++cc) {
             x->row_first.push_back((uint64_t)def_tb[dd] + cc);
             x->row_stride.push_back(def_slice[dd]);
             x->row_len.push_back(nch);
+            row_def.push_back(dd);
         }
     }
     x->num_rows = (unsigned)x->row_first.size();
     {
         std::vector<unsigned> rf(x->row_first.begin(), x->row_first.end());
-        if (upload_vec(&x->row_first_d, rf) || upload_vec(&x->row_stride_d, x->row_stride))
+        if (upload_vec(&x->row_first_d, rf) || upload_vec(&x->row_stride_d, x->row_stride) ||
+            upload_vec(&x->row_def_d, row_def))
             return -1;
     }
     x->nrec = (unsigned)prs.size();
@@ -764,7 +813,7 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
         upload_vec(&x->def_of, def_of) || upload_vec(&x->def_tb, def_tb) || upload_vec(&x->def_slice, def_slice) ||
         upload_vec(&x->def_pbegin, def_pbegin) || upload_vec(&x->def_prior, def_prior) ||
         upload_vec(&x->defs, definfo) || dmalloc(&x->G, std::max<size_t>((size_t)prs.size() * S * S, 1)) ||
-        upload_vec(&x->parents, prs) || upload_vec(&x->par_def, par_def) || dmalloc(&x->P, x->num_params) || dmalloc(&x->gP, x->num_params) ||
+        upload_vec(&x->parents, prs) || upload_vec(&x->par_def, par_def) || dmalloc(&x->P, x->num_params) ||
         dmalloc(&x->st0, VS) || dmalloc(&x->gin, VS) || dmalloc(&x->gst[0], VS) || dmalloc(&x->gst[1], VS) ||
         dmalloc(&x->w[0], ES) || dmalloc(&x->w[1], ES) || dmalloc(&x->jbar, (size_t)E * S * S) ||
         dmalloc(&x->dst0p, VS) || dmalloc(&x->dst0, VS))
@@ -1110,12 +1159,38 @@ static void enqueue_backward(bp_diff *x, hipStream_t st)
     if (nG)
         hipLaunchKernelGGL(k_parent_grad, dim3(blocks_for(nG)), dim3(256), 0, st, x->jbar, x->parents, x->G, x->nrec,
                            x->S);
-    if (x->num_params)
+}
+
+/* The gradient of the parameters (log = 0: dL/dparams into out) or of the
+ * log-parameters (log = 1: dL/dtheta, the softmax's transpose folded in),
+ * from the per-parent sums of the last backward. */
+static void launch_param_grad(bp_diff *x, int log, float *out, hipStream_t st)
+{
+    if (log && x->num_rows) {
+        switch (x->S) {
+#define BP_PGL_CASE(SS)                                                                                          \
+    case SS:                                                                                                     \
+        hipLaunchKernelGGL(k_param_grad_log<SS>, dim3(blocks_for(x->num_rows)), dim3(256), 0, st, x->G, x->dst0,  \
+                           x->row_def_d, x->defs, x->P, out, x->num_rows, x->SP);                                \
+        break;
+            BP_PGL_CASE(1)
+            BP_PGL_CASE(2)
+            BP_PGL_CASE(3)
+            BP_PGL_CASE(4)
+            BP_PGL_CASE(5)
+            BP_PGL_CASE(6)
+            BP_PGL_CASE(7)
+            BP_PGL_CASE(8)
+            BP_PGL_CASE(9)
+            BP_PGL_CASE(10)
+#undef BP_PGL_CASE
+        }
+    } else if (!log && x->num_params) {
         switch (x->S) {
 #define BP_PG_CASE(SS)                                                                                        \
     case SS:                                                                                                  \
         hipLaunchKernelGGL(k_param_grad<SS>, dim3(blocks_for(x->num_params)), dim3(256), 0, st, x->G, x->dst0, \
-                           x->def_of, x->defs, x->gP, x->num_params, x->SP);                                   \
+                           x->def_of, x->defs, out, x->num_params, x->SP);                                     \
         break;
             BP_PG_CASE(1)
             BP_PG_CASE(2)
@@ -1129,6 +1204,7 @@ static void enqueue_backward(bp_diff *x, hipStream_t st)
             BP_PG_CASE(10)
 #undef BP_PG_CASE
         }
+    }
 }
 
 extern "C" int bp_diff_backward_ex(bp_diff *x, const float *grad_beliefs, float *grad_params, float *grad_tables,
@@ -1160,9 +1236,8 @@ extern "C" int bp_diff_backward_ex(bp_diff *x, const float *grad_beliefs, float 
         it = x->bwd_graphs.emplace(key, exec).first;
     }
     HIP_TRY(hipGraphLaunch(it->second, st));
-    if (grad_params && x->num_params)
-        hipLaunchKernelGGL(k_copy, dim3(blocks_for(x->num_params)), dim3(256), 0, st, x->gP, grad_params,
-                           (size_t)x->num_params);
+    if (grad_params)
+        launch_param_grad(x, 0, grad_params, st);
     size_t SS = (size_t)x->S * x->S;
     if (grad_tables && x->E)
         hipLaunchKernelGGL(k_slot_to_edge, dim3(blocks_for((size_t)x->E * SS)), dim3(256), 0, st, x->jbar,
@@ -1187,9 +1262,7 @@ extern "C" int bp_diff_backward_log(bp_diff *x, const float *grad_beliefs, float
     }
     if (bp_diff_backward_ex(x, grad_beliefs, nullptr, nullptr, nullptr, stream))
         return -1;
-    if (x->num_rows)
-        hipLaunchKernelGGL(k_softmax_back, dim3(blocks_for(x->num_rows)), dim3(256), 0, (hipStream_t)stream, x->P,
-                           x->gP, x->row_first_d, x->row_stride_d, grad_theta, x->num_rows, x->S);
+    launch_param_grad(x, 1, grad_theta, (hipStream_t)stream);
     HIP_TRY(hipGetLastError());
     return 0;
 }
