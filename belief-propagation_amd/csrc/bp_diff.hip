@@ -50,6 +50,12 @@ using namespace bpk;
 #ifndef BP_ADJ_SLOTS_SMALL
 #define BP_ADJ_SLOTS_SMALL 2
 #endif
+/* 1: each slot's w is written at the slot's position in the source-grouped
+ * list, so a node reads its out-edges' w contiguously; 0: w in slot order,
+ * gathered through the list */
+#ifndef BP_ADJ_W_SRC
+#define BP_ADJ_W_SRC 1
+#endif
 
 namespace {
 
@@ -61,6 +67,7 @@ struct ParentRec {
 
 struct AdjArgs {
     const unsigned *in_off, *slot_src, *out_off, *out_slot;
+    const unsigned *slot_pos; /* position of each slot in the source-grouped list (out_slot's inverse) */
     const float *J;
     const uint2 *chunk;
     unsigned nchunks;
@@ -140,6 +147,30 @@ __global__ void k_initial(const float *__restrict__ P, const unsigned *__restric
     st0[t] = (tb != NONE && k < S) ? P[tb + k] : base[t];
 }
 
+/* acc += rows[o] for o in [o0, o1), in order; four rows loaded before any
+ * is added */
+template <int S, int SP>
+__device__ __forceinline__ void sum_rows_contig(const float *__restrict__ rows, unsigned o0, unsigned o1, float *acc)
+{
+    for (unsigned o = o0; o < o1; o += 4) {
+        float x[4][S];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            load_rowp<S, SP>(rows + (size_t)((o + u < o1) ? o + u : o) * SP, x[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (o + u < o1)
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    acc[k] = acc[k] + x[u][k];
+    }
+}
+
+/* the node's sum of its out-edges' w (BP_ADJ_W_SRC: stored in list order) */
+template <int S, int SP>
+__device__ __forceinline__ void sum_out_w(const float *__restrict__ w, const unsigned *__restrict__ out_slot,
+                                          unsigned o0, unsigned o1, float *acc);
+
 /* acc += rows[out_slot[o]] for o in [o0, o1), in order; four index loads,
  * then four row loads, are issued before any is added */
 template <int S, int SP>
@@ -162,6 +193,16 @@ __device__ __forceinline__ void sum_out_rows(const float *__restrict__ rows, con
                 for (int k = 0; k < S; ++k)
                     acc[k] = acc[k] + x[u][k];
     }
+}
+
+template <int S, int SP>
+__device__ __forceinline__ void sum_out_w(const float *__restrict__ w, const unsigned *__restrict__ out_slot,
+                                          unsigned o0, unsigned o1, float *acc)
+{
+    if constexpr (BP_ADJ_W_SRC)
+        sum_rows_contig<S, SP>(w, o0, o1, acc);
+    else
+        sum_out_rows<S, SP>(w, out_slot, o0, o1, acc);
 }
 
 template <int S, int STEP0, int R>
@@ -254,7 +295,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
                     }
                 }
                 if (a.w_in)
-                    sum_out_rows<S, SP>(a.w_in, a.out_slot, o0, o1, gb);
+                    sum_out_w<S, SP>(a.w_in, a.out_slot, o0, o1, gb);
                 float out[S];
                 if (STEP0) {
 #pragma unroll
@@ -350,9 +391,10 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
                     }
                     wv[j] = acc;
                 }
+                size_t wrow = BP_ADJ_W_SRC ? (size_t)a.slot_pos[s] : (size_t)s;
 #pragma unroll
                 for (int j = 0; j < SP; ++j)
-                    a.w_out[(size_t)s * SP + j] = wv[j];
+                    a.w_out[wrow * SP + j] = wv[j];
             }
         }
         wave_sync();
@@ -371,7 +413,7 @@ __global__ void k_initial_grad(const float *__restrict__ part, const float *__re
         return;
     float acc[S];
     load_rowp<S, SP>(part + (size_t)v * SP, acc);
-    sum_out_rows<S, SP>(w0, out_slot, out_off[v], out_off[v + 1], acc);
+    sum_out_w<S, SP>(w0, out_slot, out_off[v], out_off[v + 1], acc);
 #pragma unroll
     for (int k = 0; k < SP; ++k)
         out[(size_t)v * SP + k] = (k < S) ? acc[k] : 0.0f;
@@ -561,7 +603,7 @@ struct bp_diff {
     unsigned *par_def = nullptr; /* definition of each parent record */
     unsigned nrec = 0;
     unsigned *prior_tb = nullptr;
-    unsigned *out_off = nullptr, *out_slot = nullptr, *slot_edge = nullptr;
+    unsigned *out_off = nullptr, *out_slot = nullptr, *slot_pos = nullptr, *slot_edge = nullptr;
     unsigned *def_of = nullptr, *def_tb = nullptr, *def_slice = nullptr, *def_pbegin = nullptr, *def_prior = nullptr;
     DefInfo *defs = nullptr; /* the same per definition, packed for k_param_grad */
     float *G = nullptr;      /* [nrec][S][S] per-parent gradient sums */
@@ -619,7 +661,7 @@ extern "C" void bp_diff_free(bp_diff *x)
         (void)hipDeviceSynchronize();
     }
     diff_release_storage(x);
-    void *ps[] = {x->adj_chunk, x->defs, x->G, x->perm_d, x->inv_d, x->row_first_d, x->row_stride_d, x->row_def_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
+    void *ps[] = {x->adj_chunk, x->defs, x->G, x->perm_d, x->inv_d, x->row_first_d, x->row_stride_d, x->row_def_d, x->par_def, x->prior_tb, x->out_off, x->out_slot, x->slot_pos, x->slot_edge, x->def_of, x->def_tb, x->def_slice,
                   x->def_pbegin, x->def_prior, x->parents, x->P, x->st0, x->gin, x->gst[0], x->gst[1],
                   x->w[0], x->w[1], x->jbar, x->dst0p, x->dst0};
     for (void *p : ps)
@@ -791,6 +833,9 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
         for (unsigned s = 0; s < E; ++s)
             out_slot[fill[slot_src[s]]++] = s;
     }
+    std::vector<unsigned> slot_pos(E);
+    for (unsigned o = 0; o < E; ++o)
+        slot_pos[out_slot[o]] = o;
     {
         std::vector<unsigned> in_off(V + 1, 0);
         for (unsigned s = 0; s < E; ++s)
@@ -809,7 +854,7 @@ static int diff_build(bp_diff *x, const bp_bn *bn, unsigned npd, Graph_t g)
     size_t VS = (size_t)V * d->SP, ES = (size_t)E * d->SP;
     if (upload_vec(&x->perm_d, d->perm) || upload_vec(&x->inv_d, d->inv) || upload_vec(&x->prior_tb, prior_tb) ||
         upload_vec(&x->out_off, out_off) ||
-        upload_vec(&x->out_slot, out_slot) || upload_vec(&x->slot_edge, d->slot_edge) ||
+        upload_vec(&x->out_slot, out_slot) || upload_vec(&x->slot_pos, slot_pos) || upload_vec(&x->slot_edge, d->slot_edge) ||
         upload_vec(&x->def_of, def_of) || upload_vec(&x->def_tb, def_tb) || upload_vec(&x->def_slice, def_slice) ||
         upload_vec(&x->def_pbegin, def_pbegin) || upload_vec(&x->def_prior, def_prior) ||
         upload_vec(&x->defs, definfo) || dmalloc(&x->G, std::max<size_t>((size_t)prs.size() * S * S, 1)) ||
@@ -1068,6 +1113,7 @@ static void adj_step(bp_diff *x, int step0, const float *st_prev, const float *s
     a.slot_src = d->slot_src;
     a.out_off = x->out_off;
     a.out_slot = x->out_slot;
+    a.slot_pos = x->slot_pos;
     a.J = d->J;
     a.chunk = x->adj_chunk;
     a.nchunks = x->adj_nchunks;
