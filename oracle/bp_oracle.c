@@ -14,8 +14,8 @@
  *   orc_tree_run                 src/c_benchmark/main.c:205-215
  * in float on the Graph_t arrays themselves (bit-exact with the reference on
  * networks where neighbouring nodes share a state count; zero-extension
- * otherwise) and in double on a widened copy ("orc64_*", the FP64 oracle the
- * north-star tolerance is stated against).
+ * otherwise) and in double on a widened copy ("orc64_*", the FP64 oracle that
+ * the GPU's FP32 beliefs are measured against, 1e-5 max-abs).
  *
  * Pinning (each checked in tests/test_oracle.py):
  *   - golden vectors written by the compiled reference (tests/golden/), per

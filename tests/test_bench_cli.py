@@ -1,6 +1,6 @@
 """bench.py's command line without a GPU: more than one GPU is refused with
-a message (this build measures one), and the option set the driver uses
-parses."""
+a message (the benchmark measures one), and its documented options are
+listed."""
 from __future__ import annotations
 
 import subprocess
