@@ -39,6 +39,7 @@ struct bp_dev {
     float *J = nullptr;               /* [E][S*S]; mixed state counts: each slot's x*y floats, packed */
     unsigned *slot_toff = nullptr;    /* mixed state counts: [E] offset of each slot's table in J */
     std::vector<unsigned> toff;       /* the same on the host */
+    std::vector<unsigned> in_off_h;   /* [V+1] in_off on the host */
     size_t jfloats = 0;               /* floats in J */
     float *h_stage = nullptr;         /* pinned host staging for uploads and downloads */
     size_t h_stage_n = 0;             /* its capacity in floats */

@@ -554,6 +554,7 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
             d->slot_edge.push_back(g->dest_nodes_to_edges_edge_list[k]);
     }
     in_off[V] = E;
+    d->in_off_h = in_off;
     d->edge_slot.assign(E, 0);
     for (unsigned s = 0; s < E; ++s)
         d->edge_slot[d->slot_edge[s]] = s;

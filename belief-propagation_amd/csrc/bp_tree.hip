@@ -28,38 +28,49 @@
 
 using namespace bpk;
 
+/* sub-waves of at most this many nodes run several to a launch, in one
+ * workgroup */
+constexpr unsigned SMALL_WAVE = 1024;
+
+struct TreeEntry;
+
 struct bp_tree_plan {
-    unsigned *entries = nullptr;  /* node of each entry, waves concatenated */
-    unsigned *act_off = nullptr;  /* [entries+1] */
+    TreeEntry *ent = nullptr;     /* entries, waves concatenated */
     unsigned *act_slot = nullptr; /* active out-slots */
+    unsigned *wave_begin_d = nullptr;
+    unsigned char *wave_start_d = nullptr;
     std::vector<unsigned> wave_begin; /* [waves+1] entry ranges */
     std::vector<unsigned char> wave_start;
     hipGraphExec_t graph[2] = {nullptr, nullptr}; /* per message buffer it runs on */
 };
 
+/* One node of a sub-wave: its device node, its in-slot range, its range
+ * of active out-slots and its state count, in one 32-byte record (one load
+ * before the message and table reads instead of a chain of index loads). */
+struct TreeEntry {
+    unsigned v, in_b, in_e, act_b, act_e, n, pad0, pad1;
+};
+
 struct TreeArgs {
-    const unsigned *in_off;
     const unsigned char *slot_xy;
     const float *J;
     const unsigned *slot_tab; /* mixed state counts: table offsets in J (x rows of y, no padding) */
-    const unsigned char *nvars;
     const float *states;
     float *msg;
-    const unsigned *entries;
-    const unsigned *act_off;
+    const TreeEntry *ent;
     const unsigned *act_slot;
+    const unsigned *wave_begin;       /* [waves+1] entry ranges */
+    const unsigned char *wave_start;  /* [waves] 1: a start wave (sends from the node's own state) */
 };
 
 template <int S, bool MIXED>
-__global__ __launch_bounds__(256) void k_tree_wave(TreeArgs a, unsigned begin, unsigned count, int start)
+__device__ __forceinline__ void tree_entry(const TreeArgs &a, unsigned entry, int start)
 {
     constexpr int SP = Stride<S>::value;
-    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count)
-        return;
-    unsigned entry = begin + i;
-    unsigned v = a.entries[entry];
-    unsigned n = MIXED ? a.nvars[v] : S;
+    const uint4 *e4 = reinterpret_cast<const uint4 *>(a.ent + entry);
+    uint4 lo = e4[0], hi = e4[1];
+    unsigned v = lo.x, in_b = lo.y, in_e = lo.z, act_b = lo.w, act_e = hi.x;
+    unsigned n = MIXED ? hi.y : (unsigned)S;
     float buf[S], m[S];
     if (start) {
 #pragma unroll
@@ -69,14 +80,14 @@ __global__ __launch_bounds__(256) void k_tree_wave(TreeArgs a, unsigned begin, u
 #pragma unroll
         for (int k = 0; k < S; ++k)
             buf[k] = (!MIXED || (unsigned)k < n) ? 1.0f : 0.0f;
-        for (unsigned s = a.in_off[v]; s < a.in_off[v + 1]; ++s) {
+        for (unsigned s = in_b; s < in_e; ++s) {
 #pragma unroll
             for (int k = 0; k < S; ++k)
                 m[k] = a.msg[(size_t)s * SP + k];
             combine<S, MIXED>(buf, m, n);
         }
     }
-    for (unsigned q = a.act_off[entry]; q < a.act_off[entry + 1]; ++q) {
+    for (unsigned q = act_b; q < act_e; ++q) {
         unsigned s = a.act_slot[q];
         unsigned x = S, y = S;
         if (MIXED) {
@@ -99,6 +110,30 @@ __global__ __launch_bounds__(256) void k_tree_wave(TreeArgs a, unsigned begin, u
 #pragma unroll
         for (int k = 0; k < SP; ++k)
             a.msg[(size_t)s * SP + k] = (k < S) ? m[k] : 0.0f;
+    }
+}
+
+/* one sub-wave, one thread per node */
+template <int S, bool MIXED>
+__global__ __launch_bounds__(256) void k_tree_wave(TreeArgs a, unsigned begin, unsigned count, int start)
+{
+    unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count)
+        tree_entry<S, MIXED>(a, begin + i, start);
+}
+
+/* consecutive small sub-waves [w0, w1) in one workgroup: each wave's nodes,
+ * then a workgroup barrier (which orders the messages one wave writes
+ * before the next wave reads them) */
+template <int S, bool MIXED>
+__global__ __launch_bounds__(256) void k_tree_waves_small(TreeArgs a, unsigned w0, unsigned w1)
+{
+    for (unsigned w = w0; w < w1; ++w) {
+        unsigned b = a.wave_begin[w], e = a.wave_begin[w + 1];
+        int start = a.wave_start[w];
+        for (unsigned entry = b + threadIdx.x; entry < e; entry += blockDim.x)
+            tree_entry<S, MIXED>(a, entry, start);
+        __syncthreads();
     }
 }
 
@@ -236,21 +271,28 @@ struct PlanBuilder {
 
 int upload_plan(bp_dev *d, PlanBuilder &pb, bp_tree_plan **out)
 {
-    for (unsigned &v : pb.entries) /* the kernels index device nodes */
-        v = d->perm[v];
+    /* entry records in device numbering */
+    size_t ne = pb.entries.size();
+    std::vector<TreeEntry> ent(ne);
+    for (size_t i = 0; i < ne; ++i) {
+        unsigned v = d->perm[pb.entries[i]];
+        ent[i] = TreeEntry{v, d->in_off_h[v], d->in_off_h[v + 1], pb.act_off[i], pb.act_off[i + 1], d->node_n[v], 0, 0};
+    }
     bp_tree_plan *p = new bp_tree_plan();
     p->wave_begin = pb.wave_begin;
     p->wave_start = pb.wave_start;
-    if (dmalloc(&p->entries, pb.entries.size()) || dmalloc(&p->act_off, pb.act_off.size()) ||
-        dmalloc(&p->act_slot, pb.act_slot.size())) {
+    if (dmalloc(&p->ent, ne) || dmalloc(&p->act_slot, pb.act_slot.size()) ||
+        dmalloc(&p->wave_begin_d, pb.wave_begin.size()) || dmalloc(&p->wave_start_d, pb.wave_start.size())) {
         bp_tree_plan_free(p);
         return -1;
     }
-    bool ok = (pb.entries.empty() ||
-               hipMemcpy(p->entries, pb.entries.data(), pb.entries.size() * 4, hipMemcpyHostToDevice) == hipSuccess) &&
-              hipMemcpy(p->act_off, pb.act_off.data(), pb.act_off.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+    bool ok = (ne == 0 || hipMemcpy(p->ent, ent.data(), ne * sizeof(TreeEntry), hipMemcpyHostToDevice) == hipSuccess) &&
               (pb.act_slot.empty() ||
-               hipMemcpy(p->act_slot, pb.act_slot.data(), pb.act_slot.size() * 4, hipMemcpyHostToDevice) == hipSuccess);
+               hipMemcpy(p->act_slot, pb.act_slot.data(), pb.act_slot.size() * 4, hipMemcpyHostToDevice) == hipSuccess) &&
+              hipMemcpy(p->wave_begin_d, pb.wave_begin.data(), pb.wave_begin.size() * 4, hipMemcpyHostToDevice) ==
+                  hipSuccess &&
+              (pb.wave_start.empty() || hipMemcpy(p->wave_start_d, pb.wave_start.data(), pb.wave_start.size(),
+                                                  hipMemcpyHostToDevice) == hipSuccess);
     if (!ok) {
         bp_set_error("upload of the tree schedule failed");
         bp_tree_plan_free(p);
@@ -292,10 +334,12 @@ void bp_tree_plan_free(bp_tree_plan *p)
     for (hipGraphExec_t &x : p->graph)
         if (x)
             (void)hipGraphExecDestroy(x);
-    if (p->entries)
-        (void)hipFree(p->entries);
-    if (p->act_off)
-        (void)hipFree(p->act_off);
+    if (p->ent)
+        (void)hipFree(p->ent);
+    if (p->wave_begin_d)
+        (void)hipFree(p->wave_begin_d);
+    if (p->wave_start_d)
+        (void)hipFree(p->wave_start_d);
     if (p->act_slot)
         (void)hipFree(p->act_slot);
     delete p;
@@ -329,26 +373,40 @@ int bp_tree_plan_build(bp_dev *d, Graph_t g)
 static TreeArgs tree_args(bp_dev *d, const bp_tree_plan *p)
 {
     TreeArgs a;
-    a.in_off = d->in_off;
     a.slot_xy = d->slot_xy;
     a.J = d->J;
     a.slot_tab = d->mixed ? d->slot_toff : nullptr;
-    a.nvars = d->nvars;
     a.states = d->states;
     a.msg = d->msg[a_index(d)];
-    a.entries = p->entries;
-    a.act_off = p->act_off;
+    a.ent = p->ent;
     a.act_slot = p->act_slot;
+    a.wave_begin = p->wave_begin_d;
+    a.wave_start = p->wave_start_d;
     return a;
 }
 
+/* the sub-waves in order: a wave of more than SMALL_WAVE nodes is one
+ * launch, one thread per node; a run of smaller ones is one launch of one
+ * workgroup */
 template <int S, bool MIXED> static void waves_s(bp_dev *d, const bp_tree_plan *p)
 {
     TreeArgs a = tree_args(d, p);
-    for (size_t w = 0; w + 1 < p->wave_begin.size(); ++w) {
-        unsigned b = p->wave_begin[w], n = p->wave_begin[w + 1] - b;
-        hipLaunchKernelGGL((k_tree_wave<S, MIXED>), dim3((n + 255) / 256), dim3(256), 0, d->stream, a, b, n,
-                           (int)p->wave_start[w]);
+    size_t nw = p->wave_begin.size() - 1;
+    auto size = [&](size_t w) { return p->wave_begin[w + 1] - p->wave_begin[w]; };
+    for (size_t w = 0; w < nw;) {
+        if (size(w) > SMALL_WAVE) {
+            unsigned b = p->wave_begin[w], n = size(w);
+            hipLaunchKernelGGL((k_tree_wave<S, MIXED>), dim3((n + 255) / 256), dim3(256), 0, d->stream, a, b, n,
+                               (int)p->wave_start[w]);
+            ++w;
+            continue;
+        }
+        size_t w1 = w + 1;
+        while (w1 < nw && size(w1) <= SMALL_WAVE)
+            ++w1;
+        hipLaunchKernelGGL((k_tree_waves_small<S, MIXED>), dim3(1), dim3(256), 0, d->stream, a, (unsigned)w,
+                           (unsigned)w1);
+        w = w1;
     }
 }
 
