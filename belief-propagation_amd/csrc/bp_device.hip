@@ -566,7 +566,8 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
         slot_xy[s] = (unsigned char)(g->edges_x_dim[e] | (g->edges_y_dim[e] << 4));
     }
     std::vector<uint2> chunks;
-    bp_build_chunks(in_off, V, WAVE * (S <= 4 ? (unsigned)BP_SLOTS_SMALL : 1u), chunks);
+    bp_build_chunks(in_off, V, WAVE * ((S <= 4 && !(S >= 4 && BP_PLAIN_L > 1)) ? (unsigned)BP_SLOTS_SMALL : 1u),
+                    chunks);
     d->nchunks = (unsigned)chunks.size() - 1;
     /* one chunk per wave; a multiple of 8 blocks so each XCD gets one range */
     unsigned want = (d->nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
@@ -693,14 +694,26 @@ LoopyArgs bp_make_args(bp_dev *d)
 /* ev (nullable) = {start, stop}: the launch is bracketed by events taken at
  * the kernel's own begin and end (hipExtLaunchKernelGGL), so their elapsed
  * time is the kernel's duration alone, without the gap between launches. */
+template <int S> struct PlainLanes {
+    static constexpr int value = (S >= 4) ? BP_PLAIN_L : 1;
+};
+
 template <int S, bool MIXED, int MODE>
 static void launch_k(const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
 {
     dim3 grid(G), block(BLOCK);
-    if (ev)
-        hipExtLaunchKernelGGL((k_loopy<S, MIXED, MODE>), grid, block, 0, st, ev[0], ev[1], 0, a);
-    else
-        hipLaunchKernelGGL((k_loopy<S, MIXED, MODE>), grid, block, 0, st, a);
+    if constexpr (PlainLanes<S>::value > 1) {
+        constexpr int L = PlainLanes<S>::value;
+        if (ev)
+            hipExtLaunchKernelGGL((k_loopy<S, MIXED, MODE, 1, false, L>), grid, block, 0, st, ev[0], ev[1], 0, a);
+        else
+            hipLaunchKernelGGL((k_loopy<S, MIXED, MODE, 1, false, L>), grid, block, 0, st, a);
+    } else {
+        if (ev)
+            hipExtLaunchKernelGGL((k_loopy<S, MIXED, MODE>), grid, block, 0, st, ev[0], ev[1], 0, a);
+        else
+            hipLaunchKernelGGL((k_loopy<S, MIXED, MODE>), grid, block, 0, st, a);
+    }
 }
 
 template <int S, bool MIXED>

@@ -65,6 +65,11 @@ namespace bpk {
 #ifndef BP_BLOCK
 #define BP_BLOCK 256
 #endif
+/* Lanes per slot of the plain path (bp_upload handles) for four or more
+ * states: 1, 2, 4 or 8 (one for fewer states). */
+#ifndef BP_PLAIN_L
+#define BP_PLAIN_L 1
+#endif
 /* Lanes-per-slot path: 1 writes each new message as one whole row from its
  * LDS copy, one lane per slot, after the passes; 0 lets every lane store
  * the entries it formed. */
