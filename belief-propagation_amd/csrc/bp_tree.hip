@@ -45,8 +45,9 @@ struct bp_tree_plan {
 };
 
 /* One node of a sub-wave: its device node, its in-slot range, its range
- * of active out-slots and its state count, in one 32-byte record (one load
- * before the message and table reads instead of a chain of index loads). */
+ * of active out-slots (never empty) and its state count, in one 32-byte
+ * record (one load before the message and table reads instead of a chain
+ * of index loads). */
 struct TreeEntry {
     unsigned v, in_b, in_e, act_b, act_e, n, pad0, pad1;
 };
@@ -80,7 +81,21 @@ __device__ __forceinline__ void tree_entry(const TreeArgs &a, unsigned entry, in
 #pragma unroll
         for (int k = 0; k < S; ++k)
             buf[k] = (!MIXED || (unsigned)k < n) ? 1.0f : 0.0f;
-        for (unsigned s = in_b; s < in_e; ++s) {
+        unsigned s = in_b;
+        /* four rows loaded before they are combined (in slot order), so a
+         * node's in-messages are fetched together instead of one latency each */
+        for (; s + 4 <= in_e; s += 4) {
+            float m4[4][S];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int k = 0; k < S; ++k)
+                    m4[j][k] = a.msg[(size_t)(s + j) * SP + k];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                combine<S, MIXED>(buf, m4[j], n);
+        }
+        for (; s < in_e; ++s) {
 #pragma unroll
             for (int k = 0; k < S; ++k)
                 m[k] = a.msg[(size_t)s * SP + k];
@@ -271,28 +286,51 @@ struct PlanBuilder {
 
 int upload_plan(bp_dev *d, PlanBuilder &pb, bp_tree_plan **out)
 {
-    /* entry records in device numbering */
-    size_t ne = pb.entries.size();
-    std::vector<TreeEntry> ent(ne);
-    for (size_t i = 0; i < ne; ++i) {
-        unsigned v = d->perm[pb.entries[i]];
-        ent[i] = TreeEntry{v, d->in_off_h[v], d->in_off_h[v + 1], pb.act_off[i], pb.act_off[i + 1], d->node_n[v], 0, 0};
+    /* Entry records in device numbering.  The entries of one sub-wave are
+     * independent: each writes only its own out-slots and reads only slots
+     * written by earlier sub-waves.  So an entry that sends on no slot (its
+     * product would be read by nobody) is left out, and each sub-wave is
+     * ordered by the first slot its entries write: consecutive threads then
+     * read adjacent tables and write adjacent messages.  A sub-wave left
+     * empty is dropped; the order of the rest is kept. */
+    std::vector<TreeEntry> ent;
+    std::vector<unsigned> act, wbeg{0}, idx;
+    std::vector<unsigned char> wstart;
+    ent.reserve(pb.entries.size());
+    act.reserve(pb.act_slot.size());
+    for (size_t w = 0; w + 1 < pb.wave_begin.size(); ++w) {
+        idx.clear();
+        for (unsigned i = pb.wave_begin[w]; i < pb.wave_begin[w + 1]; ++i)
+            if (pb.act_off[i + 1] > pb.act_off[i])
+                idx.push_back(i);
+        std::sort(idx.begin(), idx.end(), [&](unsigned x, unsigned y) {
+            return pb.act_slot[pb.act_off[x]] < pb.act_slot[pb.act_off[y]];
+        });
+        for (unsigned i : idx) {
+            unsigned v = d->perm[pb.entries[i]];
+            unsigned ab = (unsigned)act.size();
+            act.insert(act.end(), pb.act_slot.begin() + pb.act_off[i], pb.act_slot.begin() + pb.act_off[i + 1]);
+            ent.push_back(TreeEntry{v, d->in_off_h[v], d->in_off_h[v + 1], ab, (unsigned)act.size(), d->node_n[v], 0, 0});
+        }
+        if (ent.size() != wbeg.back()) {
+            wbeg.push_back((unsigned)ent.size());
+            wstart.push_back(pb.wave_start[w]);
+        }
     }
+    size_t ne = ent.size();
     bp_tree_plan *p = new bp_tree_plan();
-    p->wave_begin = pb.wave_begin;
-    p->wave_start = pb.wave_start;
-    if (dmalloc(&p->ent, ne) || dmalloc(&p->act_slot, pb.act_slot.size()) ||
-        dmalloc(&p->wave_begin_d, pb.wave_begin.size()) || dmalloc(&p->wave_start_d, pb.wave_start.size())) {
+    p->wave_begin = wbeg;
+    p->wave_start = wstart;
+    if (dmalloc(&p->ent, ne) || dmalloc(&p->act_slot, act.size()) || dmalloc(&p->wave_begin_d, wbeg.size()) ||
+        dmalloc(&p->wave_start_d, wstart.size())) {
         bp_tree_plan_free(p);
         return -1;
     }
     bool ok = (ne == 0 || hipMemcpy(p->ent, ent.data(), ne * sizeof(TreeEntry), hipMemcpyHostToDevice) == hipSuccess) &&
-              (pb.act_slot.empty() ||
-               hipMemcpy(p->act_slot, pb.act_slot.data(), pb.act_slot.size() * 4, hipMemcpyHostToDevice) == hipSuccess) &&
-              hipMemcpy(p->wave_begin_d, pb.wave_begin.data(), pb.wave_begin.size() * 4, hipMemcpyHostToDevice) ==
-                  hipSuccess &&
-              (pb.wave_start.empty() || hipMemcpy(p->wave_start_d, pb.wave_start.data(), pb.wave_start.size(),
-                                                  hipMemcpyHostToDevice) == hipSuccess);
+              (act.empty() || hipMemcpy(p->act_slot, act.data(), act.size() * 4, hipMemcpyHostToDevice) == hipSuccess) &&
+              hipMemcpy(p->wave_begin_d, wbeg.data(), wbeg.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              (wstart.empty() ||
+               hipMemcpy(p->wave_start_d, wstart.data(), wstart.size(), hipMemcpyHostToDevice) == hipSuccess);
     if (!ok) {
         bp_set_error("upload of the tree schedule failed");
         bp_tree_plan_free(p);

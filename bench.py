@@ -107,6 +107,44 @@ def bench_adjoint(K: int = 30, c: int = 5, reps: int = 10) -> dict:
     return out
 
 
+PRESET_KIND = {
+    "c1": "seeded 1000-node binary DAG",
+    "c2": "seeded 500k-node DAG, 2-4 states",
+    "c2b": "seeded 500k-node binary DAG",
+    "c3": "seeded 1024x1024 lattice, 4 states",
+    "c5": "seeded 200k-node DAG, 3 states",
+}
+
+
+def bench_tree(reps: int = 7) -> dict:
+    """Tree BP on the seeded 1M-node tree (2 states): the start wave, the
+    up and down sweeps over the levels and the marginalisation, replayed
+    from one hipGraph; median wall time of reps runs, each synchronised."""
+    lib = B.load()
+    bn = B.generate("tree")
+    g = B.bn_to_graph(bn, levels=True)
+    lib.bp_bn_free(bn)
+    dev = B.Device(g, 0)
+    info = dev.info()
+    dev.restore()
+    dev.tree()
+    dev.sync()
+    ts = []
+    for _ in range(reps):
+        dev.restore()
+        dev.sync()
+        t0 = time.perf_counter()
+        dev.tree()
+        dev.sync()
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    out = {"workload": f"tree: {info['V']} nodes, {info['E']} message edges, {info['tree_waves']} sub-waves",
+           "tree_ms": t * 1e3, "msg_updates_per_s": info["E"] / t}
+    dev.free()
+    lib.graph_destroy(g)
+    return out
+
+
 def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: int = 3, plain_leg: bool = True) -> dict:
     """The batched workload (512 DAGs, node counts log-uniform in 200-50k,
     in-degree 0-4, states 2-8, 8 evidence sets of 3% each, 30 iterations):
@@ -195,8 +233,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--preset", default="c2b")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--forms", default="adjoint,batched",
-                    help="comma list of the further forms measured in the same run (adjoint, batched; '' for none)")
+    ap.add_argument("--forms", default="adjoint,batched,tree",
+                    help="comma list of the further forms measured in the same run (adjoint, batched, tree; '' "
+                         "for none)")
     ap.add_argument("--batch-nets", type=int, default=512, help="networks in the batched form (8 evidence sets each)")
     ap.add_argument("--no-batch-plain", action="store_true",
                     help="batched form without its plain per-network comparison")
@@ -306,6 +345,9 @@ def main():
     if "adjoint" in wanted:
         progress("adjoint form")
         forms["adjoint"] = bench_adjoint()
+    if "tree" in wanted:
+        progress("tree form")
+        forms["tree"] = bench_tree()
     if "batched" in wanted:
         forms["batched"] = bench_batched(count=args.batch_nets, plain_leg=not args.no_batch_plain)
 
@@ -322,7 +364,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{args.preset}: seeded binary DAG, init + {ITERATIONS} loopy iterations",
+        "config": {"workload": f"{args.preset}: {PRESET_KIND.get(args.preset, 'seeded network')}, "
+                               f"init + {ITERATIONS} loopy iterations",
                    "nodes": V, "arcs": arcs, "message_edges": E, "iterations": ITERATIONS,
                    "state_stride": info["stride"], "delta": "fp64 per iteration"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
