@@ -29,8 +29,16 @@
 using namespace bpk;
 
 /* sub-waves of at most this many nodes run several to a launch, in one
- * workgroup */
-constexpr unsigned SMALL_WAVE = 1024;
+ * workgroup of SMALL_BLOCK threads (1M-node tree: 1024 threads 0.252 ms,
+ * 256 threads 0.260 ms; more than 1024 nodes per small wave was slower) */
+#ifndef BP_TREE_SMALL_WAVE
+#define BP_TREE_SMALL_WAVE 1024
+#endif
+#ifndef BP_TREE_SMALL_BLOCK
+#define BP_TREE_SMALL_BLOCK 1024
+#endif
+constexpr unsigned SMALL_WAVE = BP_TREE_SMALL_WAVE;
+constexpr unsigned SMALL_BLOCK = BP_TREE_SMALL_BLOCK;
 
 struct TreeEntry;
 
@@ -141,7 +149,7 @@ __global__ __launch_bounds__(256) void k_tree_wave(TreeArgs a, unsigned begin, u
  * then a workgroup barrier (which orders the messages one wave writes
  * before the next wave reads them) */
 template <int S, bool MIXED>
-__global__ __launch_bounds__(256) void k_tree_waves_small(TreeArgs a, unsigned w0, unsigned w1)
+__global__ __launch_bounds__(SMALL_BLOCK) void k_tree_waves_small(TreeArgs a, unsigned w0, unsigned w1)
 {
     for (unsigned w = w0; w < w1; ++w) {
         unsigned b = a.wave_begin[w], e = a.wave_begin[w + 1];
@@ -442,7 +450,7 @@ template <int S, bool MIXED> static void waves_s(bp_dev *d, const bp_tree_plan *
         size_t w1 = w + 1;
         while (w1 < nw && size(w1) <= SMALL_WAVE)
             ++w1;
-        hipLaunchKernelGGL((k_tree_waves_small<S, MIXED>), dim3(1), dim3(256), 0, d->stream, a, (unsigned)w,
+        hipLaunchKernelGGL((k_tree_waves_small<S, MIXED>), dim3(1), dim3(SMALL_BLOCK), 0, d->stream, a, (unsigned)w,
                            (unsigned)w1);
         w = w1;
     }
