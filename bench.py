@@ -107,6 +107,8 @@ def bench_adjoint(K: int = 30, c: int = 5, reps: int = 10) -> dict:
     return out
 
 
+FORMS = ("adjoint", "batched", "tree")
+
 PRESET_KIND = {
     "c1": "seeded 1000-node binary DAG",
     "c2": "seeded 500k-node DAG, 2-4 states",
@@ -255,6 +257,11 @@ def main():
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU partitioning is a design only)", file=sys.stderr)
+        sys.exit(2)
+    unknown = [f for f in args.forms.split(",") if f and f not in FORMS]
+    if unknown:
+        print(f"bench.py: unknown form(s) {', '.join(unknown)} in --forms (known: {', '.join(FORMS)})",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

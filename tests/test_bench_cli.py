@@ -1,6 +1,6 @@
 """bench.py's command line without a GPU: more than one GPU is refused with
-a message (the benchmark measures one), and its documented options are
-listed."""
+a message (the benchmark measures one), so is an unknown form name, and its
+documented options are listed."""
 from __future__ import annotations
 
 import subprocess
@@ -26,3 +26,9 @@ def test_help_lists_the_contract_options():
     assert r.returncode == 0
     for opt in ("--gpus", "--steps", "--warmup", "--forms", "--no-cpu", "--batch-nets", "--no-batch-plain"):
         assert opt in r.stdout
+
+
+def test_unknown_form_is_refused():
+    r = run("--forms", "adjoint,treee")
+    assert r.returncode == 2
+    assert "treee" in r.stderr and r.stdout == ""
