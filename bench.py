@@ -109,6 +109,47 @@ def bench_adjoint(K: int = 30, c: int = 5, reps: int = 10) -> dict:
 
 FORMS = ("adjoint", "batched", "tree")
 
+def reference_cpu(bn, threads: int, gpu_states) -> dict | None:
+    """The reference's own OpenMP node loop (oracle/_ref/ref_harness_omp,
+    compiled from the reference's sources) on the same graph, rebuilt through
+    the reference's graph API from a replay file: init_previous_edge and
+    ITERATIONS x loopy_propagate_one_iteration, timed inside the harness.
+    None when the harness is not built; an "error" entry when it fails."""
+    exe = ROOT / "oracle" / "_ref" / "ref_harness_omp"
+    if not exe.exists():
+        return None
+    import subprocess
+    import tempfile
+
+    import ref_files as RF
+
+    lib = B.load()
+    h = B.bn_to_graph(bn)
+    try:
+        E = h.contents.current_num_edges
+        with tempfile.TemporaryDirectory() as d:
+            rp, out = Path(d) / "graph.replay", Path(d) / "out.bin"
+            RF.write_replay(h, rp)
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            t0 = time.perf_counter()
+            subprocess.run([str(exe), "replay", str(rp), "time", str(ITERATIONS), "1", str(out)], check=True,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, env=env, timeout=600)
+            wall = time.perf_counter() - t0
+            r = RF.read_tagged(out)
+    except (OSError, subprocess.SubprocessError) as exc:
+        return {"kind": "reference", "error": str(exc)[:200]}
+    finally:
+        lib.graph_destroy(h)
+    best, best_iter = (float(x) for x in r["seconds"])
+    states = r["states"].astype(np.float32).reshape(-1, 10)
+    return {"value": E * ITERATIONS / best, "unit": "msg-updates/s", "cores": threads, "kind": "reference",
+            "iterations_only_value": E * ITERATIONS / best_iter,
+            "sample": f"the reference's OpenMP node loop compiled from its own sources: the same graph rebuilt "
+                      f"through its graph API, then init_previous_edge + {ITERATIONS} iterations once "
+                      f"({best:.2f} s on {threads} threads; {wall:.1f} s with the rebuild)",
+            "parity_vs_gpu": bool(states.tobytes() == np.ascontiguousarray(gpu_states).tobytes())}
+
+
 PRESET_KIND = {
     "c1": "seeded 1000-node binary DAG",
     "c2": "seeded 500k-node DAG, 2-4 states",
@@ -353,6 +394,8 @@ def main():
                "serial_value": E * ITERATIONS / t_ser,
                "serial_iterations_only_value": E * ITERATIONS / t_ser_iter,
                "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+        progress(f"reference CPU path: {threads} threads")
+        cpu["reference"] = reference_cpu(bn, threads, gpu_states)
 
     forms = {}
     wanted = [f for f in args.forms.split(",") if f]

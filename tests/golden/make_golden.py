@@ -27,10 +27,10 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT / "belief-propagation_amd"))
-sys.path.insert(0, str(Path(__file__).resolve().parent))
+sys.path.insert(0, str(ROOT / "oracle"))
 
 import bp_ctypes as B  # noqa: E402
-import refdump  # noqa: E402
+from ref_files import read_tagged, write_replay  # noqa: E402,F401
 
 OUT = Path(__file__).resolve().parent
 SCRATCH = OUT / "_scratch"
@@ -46,7 +46,7 @@ def harness(kind: str, path, action: list[str], fp64=False, omp=False, env=None)
     if env:
         e.update(env)
     subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, env=e)
-    return refdump.read(out)
+    return read_tagged(out)
 
 
 def compact(a: np.ndarray, count: int, width: int) -> np.ndarray:
@@ -82,42 +82,6 @@ def struct_arrays(d: dict) -> dict:
 
 def struct_digest(d: dict) -> dict:
     return {f"sha_{k}": np.array(sha(v)) for k, v in struct_arrays(d).items()}
-
-
-def write_replay(g, path):
-    """Structure file for the harness's replay source (see ref_harness.c)."""
-    G = g.contents
-    V, E = G.current_num_vertices, G.current_num_edges
-    nv = B.view_u32(G.node_num_vars, V)
-    st = B.view_f32(G.node_states, V * 10).reshape(V, 10)
-    obs = B.view_bytes(G.observed_nodes, V)
-    names = B.view_bytes(G.node_names, V * 50).reshape(V, 50)
-    src, dst = B.view_u32(G.edges_src_index, E), B.view_u32(G.edges_dest_index, E)
-    xs, ys = B.view_u32(G.edges_x_dim, E), B.view_u32(G.edges_y_dim, E)
-    J = B.view_f32(G.edges_joint_probabilities, E * 100).reshape(E, 10, 10)
-    with open(path, "wb") as f:
-        f.write(np.array([V, E], dtype=np.uint32).tobytes())
-        if V and (nv == nv[0]).all():
-            n = int(nv[0])
-            rec = np.zeros(V, dtype=[("n", "<u4"), ("obs", "u1"), ("st", "<f4", (n,)), ("name", "u1", (50,))])
-            rec["n"], rec["obs"], rec["st"], rec["name"] = nv, obs, st[:, :n], names
-            f.write(rec.tobytes())
-            x = n
-            erec = np.zeros(E, dtype=[("hdr", "<u4", (4,)), ("J", "<f4", (x * x,))])
-            erec["hdr"] = np.stack([src, dst, xs, ys], axis=1)
-            erec["J"] = J[:, :x, :x].reshape(E, x * x)
-            f.write(erec.tobytes())
-            return
-        for v in range(V):
-            n = int(nv[v])
-            f.write(np.array([n], dtype=np.uint32).tobytes())
-            f.write(bytes([int(obs[v])]))
-            f.write(st[v, :n].astype(np.float32).tobytes())
-            f.write(names[v].tobytes())
-        for e in range(E):
-            x, y = int(xs[e]), int(ys[e])
-            f.write(np.array([src[e], dst[e], x, y], dtype=np.uint32).tobytes())
-            f.write(J[e, :x, :y].astype(np.float32).tobytes())
 
 
 def write_levels(g, path):
