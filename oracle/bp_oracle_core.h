@@ -14,7 +14,23 @@
  * uninitialised when neighbouring nodes have different state counts
  * (graph.c:856 vs :899, :103-104 vs :455) -- are defined here as 0.0
  * (zero-extension).
+ *
+ * ORC_PLANT_PAD_ONE, ORC_PLANT_STALE and ORC_PLANT_NORM_STRIDE build
+ * deliberately wrong variants (out-of-range reads as 1.0; out-of-range reads
+ * of whatever the padded row holds; normalisation over the whole
+ * MAX_STATES-wide row).  liboracle.so never sets them: the variants exist so
+ * the oracle's tests can show that they reject each of these mistakes.
  */
+
+#ifndef ORC_OUT_OF_RANGE
+#if defined(ORC_PLANT_PAD_ONE)
+#define ORC_OUT_OF_RANGE(row, i) 1.0
+#elif defined(ORC_PLANT_STALE)
+#define ORC_OUT_OF_RANGE(row, i) (row)[i]
+#else
+#define ORC_OUT_OF_RANGE(row, i) 0.0
+#endif
+#endif
 
 /* FN(x) names the instantiation; R is the real type. */
 
@@ -41,6 +57,9 @@ static inline unsigned int FN(end)(const FN(view) *g, const unsigned int *nodes,
 
 static void FN(normalise)(R *v, unsigned int n)
 {
+#ifdef ORC_PLANT_NORM_STRIDE
+    n = MAX_STATES;
+#endif
     R s = (R)0.0;
     for (unsigned int i = 0; i < n; ++i)
         s = s + v[i];
@@ -59,12 +78,16 @@ static void FN(send)(const FN(view) *g, const R *b, unsigned int nb, unsigned in
     for (unsigned int i = 0; i < x; ++i) {
         R acc = (R)0.0;
         for (unsigned int j = 0; j < y; ++j) {
-            R bj = (j < nb) ? b[j] : (R)0.0;
+            R bj = (j < nb) ? b[j] : (R)ORC_OUT_OF_RANGE(b, j);
             acc = acc + J[MAX_STATES * i + j] * bj;
         }
         out[i] = acc;
         s = s + acc;
     }
+#ifdef ORC_PLANT_NORM_STRIDE
+    for (unsigned int i = x; i < MAX_STATES; ++i)
+        s = s + out[i];
+#endif
     if (s <= (R)0.0)
         s = (R)1.0;
     for (unsigned int i = 0; i < x; ++i)
@@ -75,7 +98,7 @@ static void FN(send)(const FN(view) *g, const R *b, unsigned int nb, unsigned in
 static inline void FN(combine)(R *buf, unsigned int n, const R *msg, unsigned int x)
 {
     for (unsigned int k = 0; k < n; ++k) {
-        R m = (k < x) ? msg[k] : (R)0.0;
+        R m = (k < x) ? msg[k] : (R)ORC_OUT_OF_RANGE(msg, k);
         if (m == m)
             buf[k] = buf[k] * m;
     }

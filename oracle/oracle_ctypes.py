@@ -17,6 +17,11 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 _lib = None
+_plants: dict = {}
+
+# deliberately wrong builds of the oracle (ORC_PLANT_* in bp_oracle_core.h),
+# loaded only by the tests that show the checks reject them
+PLANTS = ("pad_one", "stale", "norm_stride")
 
 
 def build() -> None:
@@ -29,7 +34,23 @@ def load():
         return _lib
     if not LIB_PATH.exists():
         build()
-    lib = C.CDLL(str(LIB_PATH))
+    _lib = _bind(C.CDLL(str(LIB_PATH)))
+    return _lib
+
+
+def load_plant(name: str):
+    """A planted-mistake build of the oracle (built on first use)."""
+    if name not in PLANTS:
+        raise ValueError(f"unknown planted oracle {name!r}")
+    if name not in _plants:
+        path = HERE / f"liboracle_plant_{name}.so"
+        if not path.exists():
+            subprocess.run(["make", "-s", "-C", str(HERE), "plants"], check=True)
+        _plants[name] = _bind(C.CDLL(str(path)))
+    return _plants[name]
+
+
+def _bind(lib):
     vp = C.c_void_p
     sig = {
         "orc_init_previous_edge": (None, [vp]),
@@ -57,7 +78,6 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
     return lib
 
 
@@ -65,12 +85,12 @@ def _vp(g):
     return C.cast(g, C.c_void_p)
 
 
-def init_previous_edge(g):
-    load().orc_init_previous_edge(_vp(g))
+def init_previous_edge(g, lib=None):
+    (lib or load()).orc_init_previous_edge(_vp(g))
 
 
-def loopy_iteration(g, threads: int = 1):
-    lib = load()
+def loopy_iteration(g, threads: int = 1, lib=None):
+    lib = lib or load()
     if threads > 1:
         lib.orc_loopy_iteration_omp(_vp(g), threads)
     else:
@@ -95,8 +115,8 @@ def delta64(g) -> float:
     return float(load().orc_delta64(_vp(g)))
 
 
-def tree_run(g):
-    load().orc_tree_run(_vp(g))
+def tree_run(g, lib=None):
+    (lib or load()).orc_tree_run(_vp(g))
 
 
 class FP64:

@@ -305,7 +305,11 @@ int main(int argc, char **argv)
         put_padded("states", g->node_states, g->current_num_vertices, g->node_num_vars);
         dump_messages(g, "cur", "prev");
     } else if (strcmp(action, "tree") == 0) {
-        if (argc > 5) {
+        if (argc > 5 && strcmp(argv[4], "reflevels") == 0) {
+            /* the reference's own level scan (graph.c:806-827), with no
+             * structure dump: used on inputs too large to dump whole */
+            init_levels_to_nodes(g);
+        } else if (argc > 5) {
             FILE *lf = fopen(argv[4], "rb");
             assert(lf);
             assert(fread(&g->num_levels, 4, 1, lf) == 1);

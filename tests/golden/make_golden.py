@@ -237,6 +237,10 @@ def large_loopy(name, preset, iters, sample=2000, via="xml", **kw):
 
 
 def large_tree(name, preset, sample=2000, **kw):
+    """Tree BP on a large generated tree, levels from the reference's own
+    init_levels_to_nodes (graph.c:806-827; its membership scan is quadratic,
+    about 15 minutes at 1M nodes).  The build's levels are computed too and
+    must equal the reference's; both digests are stored."""
     print(name)
     bn, g, xml = generated_graph(preset, **kw)
     lib = B.load()
@@ -245,23 +249,37 @@ def large_tree(name, preset, sample=2000, **kw):
     lib.init_levels_to_nodes(g)
     G = g.contents
     V, E = G.current_num_vertices, G.current_num_edges
-    lv = SCRATCH / f"{name}.levels"
-    write_levels(g, lv)
+    build_levels = sha(B.view_u32(G.levels_to_nodes, 2 * V).copy())
     replay = SCRATCH / f"{name}.replay"
     write_replay(g, replay)
-    t = harness("replay", replay, ["tree", str(lv)])
+    t = harness("replay", replay, ["tree", "reflevels"])
+    # slots [num_levels, V) are never written by the reference (malloc'd)
+    ref_lv = t["levels_to_nodes"].copy()
+    ref_lv[int(t["num_levels"][0]):V] = 0
+    if build_levels != sha(ref_lv):
+        sys.exit(f"{name}: the build's levels differ from the reference's init_levels_to_nodes")
+    t["levels_to_nodes"] = ref_lv
     S = 2
     st = compact(t["states"], V, S)
     arrays = {"xml_sha256": np.array(file_sha(xml)), "preset": np.array(preset),
               "overrides": np.array(repr(sorted(kw.items()))),
               "V": np.array(V), "E": np.array(E), "num_levels": t["num_levels"],
               "sha_levels": np.array(sha(t["levels_to_nodes"])),
+              "levels_source": np.array("the reference's init_levels_to_nodes (graph.c:806-827)"),
               "sha_states": np.array(sha(st)),
               "sha_tree_messages": np.array(sha(compact(t["tree_messages"], E, S)))}
     rng = np.random.default_rng(11)
     idx = np.sort(rng.choice(V, size=min(sample, V), replace=False)).astype(np.uint32)
     arrays["sample_nodes"] = idx
     arrays["sample_states"] = st[idx]
+    # the -Dfloat=double graph core on the same tree and levels
+    lv = SCRATCH / f"{name}.levels"
+    write_levels(g, lv)
+    t64 = harness("replay", replay, ["tree", str(lv)], fp64=True)
+    st64 = compact(t64["states"], V, S)
+    arrays["fp64_sha_states"] = np.array(sha(st64))
+    arrays["fp64_sample_states"] = st64[idx]
+    arrays["fp32_vs_fp64_maxabs"] = np.array(np.abs(st.astype(np.float64) - st64).max())
     save(name, arrays)
 
 

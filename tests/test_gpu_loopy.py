@@ -129,6 +129,42 @@ def test_until_bitwise(lib, case):
         lib.graph_destroy(g)
 
 
+@pytest.mark.parametrize("path", ["handle", "dropin"])
+def test_mixed_states_hand_worked(lib, orc, path):
+    """Zero-extension on the GPU against exact hand-worked values
+    (tests/mixed_hand.py: a 2-state node sending on a 2x3 edge, a 3-state
+    node combining a 2-entry message, the belief update over it; every
+    padded slot filled with 7.0), and bit for bit against the oracle."""
+    import mixed_hand as MH
+
+    print(f"mixed chain, {path}", flush=True)
+    exp = MH.chain_expected()
+    g = MH.chain_graph()
+    h = MH.chain_graph()
+    if path == "handle":
+        dev = B.Device(g)
+        dev.init()
+        dev.download(g)
+    else:
+        lib.init_previous_edge(g)
+    assert MH.check_chain_init(g, exp) == []
+    orc.init_previous_edge(h)
+    assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
+    if path == "handle":
+        dev.iterate(1)
+        dev.download(g)
+        dev.free()
+    else:
+        lib.loopy_propagate_one_iteration(g)
+    assert MH.check_chain_iteration(g, exp) == []
+    orc.loopy_iteration(h)
+    assert same_bits(B.states(g), B.states(h))
+    assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
+    assert same_bits(B.messages(g, "cur"), B.messages(h, "cur"))
+    lib.graph_destroy(g)
+    lib.graph_destroy(h)
+
+
 def test_mixed_state_counts_match_oracle(lib, orc):
     """Different state counts per node: zero-extension, defined by the oracle."""
     bn = B.generate("c1", states_hi=4, seed=1002)

@@ -76,3 +76,37 @@ def test_tree_mixed_states_match_oracle(lib, orc):
     lib.graph_destroy(g)
     lib.graph_destroy(h)
     lib.bp_bn_free(bn)
+
+
+@pytest.mark.parametrize("path", ["handle", "dropin"])
+def test_tree_mixed_states_hand_worked(lib, orc, path):
+    """Zero-extension on the GPU against exact hand-worked values
+    (tests/mixed_hand.py: a 3-state node combining 2-entry messages and
+    sending on a 3x3 edge, every padded slot filled with 7.0), and bit for
+    bit against the oracle."""
+    import mixed_hand as MH
+
+    print(f"mixed star, {path}", flush=True)
+    exp = MH.star_expected()
+    g = MH.star_graph()
+    if path == "handle":
+        dev = B.Device(g)
+        dev.tree()
+        dev.download(g)
+        dev.free()
+    else:
+        L = g.contents.num_levels
+        lib.propagate_using_levels_start(g)
+        for i in range(1, L - 1):
+            lib.propagate_using_levels(g, i)
+        lib.reset_visited(g)
+        for i in range(L - 1, 0, -1):
+            lib.propagate_using_levels(g, i)
+        lib.marginalize(g)
+    assert MH.check_star(g, exp) == []
+    h = MH.star_graph()
+    orc.tree_run(h)
+    assert same_bits(B.messages(g, "tree"), B.messages(h, "tree"))
+    assert same_bits(B.states(g), B.states(h))
+    lib.graph_destroy(g)
+    lib.graph_destroy(h)

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import bp_ctypes as B
+import mixed_hand as MH
 from conftest import (SMALL_LOOPY_CASES, TREE_CASES, bits_report, golden, graph_for_case, messages_of,
                       same_bits, states_of)
 
@@ -262,6 +263,61 @@ def test_mixed_state_counts_zero_extension(lib, orc):
     lib.graph_destroy(g)
     lib.graph_destroy(h)
     lib.bp_bn_free(bn)
+
+
+def test_mixed_states_hand_worked_loopy(lib, orc):
+    """Zero-extension pinned by hand: a 2-state node sending on a 2x3 edge
+    (graph.c:896-903, the third buffer entry reads 0), a 3-state node
+    combining a 2-entry message (graph.c:450-459, the third entry reads 0)
+    and the loopy belief update over it (graph.c:985-1024), against exact
+    values worked out in tests/mixed_hand.py, with every padded slot
+    filled with 7.0; serial and OpenMP node loops."""
+    exp = MH.chain_expected()
+    for threads in (1, 2):
+        g = MH.chain_graph()
+        orc.init_previous_edge(g)
+        assert MH.check_chain_init(g, exp) == []
+        orc.loopy_iteration(g, threads=threads)
+        assert MH.check_chain_iteration(g, exp) == []
+        lib.graph_destroy(g)
+
+
+def test_mixed_states_hand_worked_tree(lib, orc):
+    """Zero-extension in tree BP (graph.c:461-538, 540-596): a 3-state node
+    combining two 2-entry messages and sending on a 3x3 edge, against exact
+    hand-worked values (tests/mixed_hand.py)."""
+    exp = MH.star_expected()
+    g = MH.star_graph()
+    G = g.contents
+    assert G.num_levels == exp["num_levels"]
+    assert list(B.view_u32(G.levels_to_nodes, 2 * G.current_num_vertices)[G.current_num_vertices:]) == exp["levels"]
+    orc.tree_run(g)
+    assert MH.check_star(g, exp) == []
+    lib.graph_destroy(g)
+
+
+@pytest.mark.parametrize("plant", ["pad_one", "stale", "norm_stride"])
+def test_mixed_hand_worked_rejects_planted_mistakes(lib, orc, plant):
+    """Each plausible mistake in the mixed-state branch, planted in a copy of
+    the oracle (ORC_PLANT_* in oracle/bp_oracle_core.h) -- reads beyond a
+    state count padded with 1.0, reads of whatever the padded row holds,
+    normalisation over the padded row width -- fails every hand-worked
+    check: the init messages, the first iteration and the tree."""
+    pl = orc.load_plant(plant)
+    exp = MH.chain_expected()
+    g = MH.chain_graph()
+    orc.init_previous_edge(g, lib=pl)
+    assert MH.check_chain_init(g, exp) != []
+    lib.graph_destroy(g)
+    g = MH.chain_graph()
+    orc.init_previous_edge(g)           # correct init, then the planted iteration
+    orc.loopy_iteration(g, lib=pl)
+    assert MH.check_chain_iteration(g, exp) != []
+    lib.graph_destroy(g)
+    g = MH.star_graph()
+    orc.tree_run(g, lib=pl)
+    assert MH.check_star(g, MH.star_expected()) != []
+    lib.graph_destroy(g)
 
 
 @pytest.mark.parametrize("case", ["example", "dog_xml", "c1", "lattice16", "c4net_small"])
