@@ -174,6 +174,7 @@ _SIGNATURES = {
     "bp_loopy_init": (C.c_int, [C.c_void_p]),
     "bp_restore_states": (C.c_int, [C.c_void_p]),
     "bp_loopy_iterate": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_double)]),
+    "bp_loopy_iterate_noalloc": (C.c_int, [C.c_void_p, C.c_uint, C.POINTER(C.c_double)]),
     "bp_loopy_until": (C.c_int, [C.c_void_p, C.c_float, C.c_uint, C.POINTER(C.c_uint)]),
     "bp_edge_iterate": (C.c_int, [C.c_void_p, C.c_uint]),
     "bp_edge_until": (C.c_int, [C.c_void_p, C.c_float, C.c_uint, C.POINTER(C.c_uint)]),

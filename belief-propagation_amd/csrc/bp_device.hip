@@ -862,6 +862,82 @@ extern "C" int bp_loopy_iterate(bp_dev *d, unsigned int n, double *deltas)
     return 0;
 }
 
+/* ---- the allocation-free iteration entry point ---------------------------
+ * bp_loopy_iterate_noalloc: the kernels of bp_loopy_iterate launched one by
+ * one (no HIP graph is created), with the per-wave FP64 delta partials of
+ * the running iteration and the per-iteration deltas in `scratch`, a fixed
+ * array of the library, instead of the handle's buffers.  Each GPU holds its
+ * own copy of the array; its device address is looked up once per device
+ * (hipGetSymbolAddress, with that device current) and only that address is
+ * passed to kernels.  Calls that overlap on one GPU share the array and
+ * race.  Nothing in it needs clearing: every wave writes its partial each
+ * iteration and k_finish writes each delta before it is read. */
+static constexpr size_t NOALLOC_SCRATCH_DOUBLES = (4u << 20) / sizeof(double); /* 4 MiB */
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static double *noalloc_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex noalloc_mu;
+
+/* The device address of `scratch` on `device` (which must be current). */
+static int noalloc_scratch(int device, double **out)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) {
+        bp_set_error("bp_loopy_iterate_noalloc: device %d is beyond the %d supported", device, NOALLOC_MAX_DEVICES);
+        return -1;
+    }
+    std::lock_guard<std::mutex> lock(noalloc_mu);
+    if (!noalloc_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            bp_set_error("bp_loopy_iterate_noalloc: hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                         hipGetErrorString(e));
+            return -1;
+        }
+        noalloc_ptr[device] = (double *)p;
+    }
+    *out = noalloc_ptr[device];
+    return 0;
+}
+
+extern "C" int bp_loopy_iterate_noalloc(bp_dev *d, unsigned int n, double *deltas)
+{
+    bp_clear_error();
+    if (!d) {
+        bp_set_error("bp_loopy_iterate_noalloc: NULL handle");
+        return -1;
+    }
+    if ((size_t)d->NP + n > NOALLOC_SCRATCH_DOUBLES) {
+        bp_set_error("bp_loopy_iterate_noalloc: %u delta partials per iteration + %u deltas exceed the fixed "
+                     "scratch of %zu doubles (4 MiB); use bp_loopy_iterate", d->NP, n, NOALLOC_SCRATCH_DOUBLES);
+        return -1;
+    }
+    HIP_TRY(hipSetDevice(d->device));
+    double *sc = nullptr;
+    if (noalloc_scratch(d->device, &sc))
+        return -1;
+    double *partials = sc, *ds = sc + d->NP;
+    if (ensure_buffer(d))
+        return -1;
+    for (unsigned i = 0; i < n; ++i) {
+        LoopyArgs a = bp_make_args(d);
+        a.partials = partials;
+        launch(d, ITER, a);
+        hipLaunchKernelGGL(k_finish, dim3(1), dim3(BLOCK), 0, d->stream, partials, d->NP, 1u, ds + i, nullptr,
+                           nullptr, 0.0f, 0, 0u);
+        d->latest ^= 1;
+        d->bcur ^= 1;
+        d->swaps += 1;
+    }
+    HIP_TRY(hipGetLastError());
+    if (deltas && n) {
+        HIP_TRY(hipMemcpyAsync(deltas, ds, n * sizeof(double), hipMemcpyDeviceToHost, d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+    }
+    return 0;
+}
+
 /* Buffer roles while a run is being enqueued.  ITER reads msg[latest] and
  * writes msg[latest^1], then the host's pointers swap (graph.c:1179-1182):
  * latest flips and a swap is owed.  EDGE copies current into previous and

@@ -100,6 +100,20 @@ int bp_loopy_init(bp_dev *d);
  * The launches are replayed from a cached HIP graph. */
 int bp_loopy_iterate(bp_dev *d, unsigned int n, double *deltas);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): n loopy iterations with the results of
+ * bp_loopy_iterate, bit for bit, that call no hipMalloc, hipFree or other
+ * allocating routine and create no HIP graph.  The FP64 delta partials and
+ * deltas live in a fixed 4 MiB device array of the library, one per GPU:
+ * the call fails with an error when the handle's delta partials per
+ * iteration (one per wave of its grid: about one per 64-128 message edges)
+ * plus n exceed 524,288 doubles, i.e. from about 30M message edges on.
+ * Calls that overlap on
+ * one GPU share that array and race.  deltas (nullable) receives the n FP64
+ * deltas; without it the call returns without waiting for the GPU. */
+int bp_loopy_iterate_noalloc(bp_dev *d, unsigned int n, double *deltas);
+
 /* Like loopy_propagate_until: runs until the stop test fires or
  * max_iterations; *result receives the reference's return value. */
 int bp_loopy_until(bp_dev *d, float convergence, unsigned int max_iterations, unsigned int *result);
