@@ -11,9 +11,12 @@ current_num_edges).
 Prints one JSON line (see the keys below).  The roofline object covers the
 per-iteration kernel; its duration is measured with HIP events on the
 library's stream, and its algorithmic bytes are those the iteration must
-move at minimum.  The CPU baseline times the oracle (the C restatement of
-the reference's OpenMP node loop, bit-identical to it) on this host, on the
-same network, and checks that the GPU beliefs equal it.
+move at minimum.  The CPU baseline times the reference's own OpenMP node
+loop (compiled from its sources into oracle/_ref/, run as a subprocess
+pinned with OMP_PROC_BIND=close on the physical cores, and once serially)
+on the same network, and checks that the GPU beliefs equal it; the oracle's
+C restatement of that loop is timed beside it.  The GPU beliefs are also
+compared with the FP64 oracle over all nodes (fp64_maxabs).
 """
 from __future__ import annotations
 
@@ -69,6 +72,22 @@ def cpu_threads() -> int:
         return os.cpu_count() or 1
 
 
+def physical_cores() -> int:
+    """Distinct (package, core) pairs among the CPUs this process may use."""
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        return cpu_threads()
+    cores = set()
+    for c in cpus:
+        t = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+        try:
+            cores.add((t.joinpath("physical_package_id").read_text().strip(), t.joinpath("core_id").read_text().strip()))
+        except OSError:
+            return len(cpus)
+    return max(1, len(cores))
+
+
 def bench_adjoint(K: int = 30, c: int = 5, reps: int = 10) -> dict:
     """The differentiated workload (200k-node 3-state DAG, log-CPT
     parameters ~N(0,1) softmaxed per parent configuration, 5% evidence,
@@ -110,10 +129,12 @@ def bench_adjoint(K: int = 30, c: int = 5, reps: int = 10) -> dict:
 FORMS = ("adjoint", "batched", "tree")
 
 def reference_cpu(bn, threads: int, gpu_states) -> dict | None:
-    """The reference's own OpenMP node loop (oracle/_ref/ref_harness_omp,
-    compiled from the reference's sources) on the same graph, rebuilt through
-    the reference's graph API from a replay file: init_previous_edge and
-    ITERATIONS x loopy_propagate_one_iteration, timed inside the harness.
+    """The reference's own node loop (oracle/_ref/ref_harness_omp, compiled
+    from the reference's sources with -fopenmp) on the same graph, rebuilt
+    through the reference's graph API from a replay file:
+    init_previous_edge and ITERATIONS x loopy_propagate_one_iteration, timed
+    inside the harness.  The subprocess runs `threads` OpenMP threads pinned
+    with OMP_PROC_BIND=close, OMP_PLACES=cores (1 thread: the serial run).
     None when the harness is not built; an "error" entry when it fails."""
     exe = ROOT / "oracle" / "_ref" / "ref_harness_omp"
     if not exe.exists():
@@ -130,7 +151,7 @@ def reference_cpu(bn, threads: int, gpu_states) -> dict | None:
         with tempfile.TemporaryDirectory() as d:
             rp, out = Path(d) / "graph.replay", Path(d) / "out.bin"
             RF.write_replay(h, rp)
-            env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+            env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
             t0 = time.perf_counter()
             subprocess.run([str(exe), "replay", str(rp), "time", str(ITERATIONS), "1", str(out)], check=True,
                            stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, env=env, timeout=600)
@@ -146,7 +167,7 @@ def reference_cpu(bn, threads: int, gpu_states) -> dict | None:
             "iterations_only_value": E * ITERATIONS / best_iter,
             "sample": f"the reference's OpenMP node loop compiled from its own sources: the same graph rebuilt "
                       f"through its graph API, then init_previous_edge + {ITERATIONS} iterations once "
-                      f"({best:.2f} s on {threads} threads; {wall:.1f} s with the rebuild)",
+                      f"({best:.2f} s on {threads} threads pinned close to cores; {wall:.1f} s with the rebuild)",
             "parity_vs_gpu": bool(states.tobytes() == np.ascontiguousarray(gpu_states).tobytes())}
 
 
@@ -212,7 +233,8 @@ def bench_batched(count: int = 512, evidence_sets: int = 8, K: int = 30, reps: i
     build_s = time.perf_counter() - t0
     batch.upload(0)
     progress("batched form: uploaded; tuning")
-    batch.tune(3, str(ROOT / "profiles" / "batch_shapes.json"))
+    (ROOT / "_build").mkdir(exist_ok=True)
+    batch.tune(3, str(ROOT / "_build" / "batch_shapes.json"))
     shape = batch.config()
 
     def step():
@@ -365,6 +387,7 @@ def main():
 
     cpu = None
     parity = None
+    fp64_maxabs = None
     if not args.no_cpu:
         import oracle_ctypes as O
 
@@ -393,8 +416,14 @@ def main():
                 "iterations_only_value": E * ITERATIONS / t_iter,
                 "serial_value": E * ITERATIONS / t_ser,
                 "serial_iterations_only_value": E * ITERATIONS / t_ser_iter}
-        progress(f"reference CPU path: {threads} threads")
-        ref = reference_cpu(bn, threads, gpu_states)
+        cores = physical_cores()
+        progress(f"reference CPU path: {cores} threads pinned to physical cores, then 1")
+        ref = reference_cpu(bn, cores, gpu_states)
+        ref1 = reference_cpu(bn, 1, gpu_states)
+        if ref is not None and ref1 is not None and "error" not in ref1:
+            ref["serial_value"] = ref1["value"]
+            ref["serial_iterations_only_value"] = ref1["iterations_only_value"]
+            ref["serial_parity_vs_gpu"] = ref1["parity_vs_gpu"]
         # the reference's own loop is the baseline where it ran and agreed;
         # the oracle's restatement is reported beside it
         if ref is not None and "error" not in ref and ref["parity_vs_gpu"]:
@@ -403,7 +432,17 @@ def main():
             cpu = dict(port, reference=ref)
             if ref is not None and "parity_vs_gpu" in ref:
                 parity = parity and ref["parity_vs_gpu"]
-        cpu.update({"nproc": os.cpu_count(), "cpu_model": cpu_model()})
+        cpu.update({"nproc": os.cpu_count(), "affinity_cpus": threads, "physical_cores": cores,
+                    "cpu_model": cpu_model()})
+        progress("FP64 oracle over all nodes")
+        h64 = B.bn_to_graph(bn)
+        f64 = O.FP64(h64)
+        f64.init()
+        f64.iterate(ITERATIONS, threads=threads)
+        S = info["S"]
+        fp64_maxabs = float(np.abs(gpu_states[:, :S].astype(np.float64) - f64.states()[:, :S]).max())
+        del f64
+        lib.graph_destroy(h64)
 
     forms = {}
     wanted = [f for f in args.forms.split(",") if f]
@@ -442,6 +481,10 @@ def main():
                      "note": "outside the timed region: bp_upload of the host graph (node order, chunk lists, "
                              "copies to HBM) and bp_download of beliefs and both message arrays into it"},
         "parity_vs_cpu": parity,
+        "accuracy": {"fp64_maxabs": fp64_maxabs,
+                     "note": "all-node max |GPU belief - FP64 oracle belief| after the same run; the GPU is bit-equal "
+                             "to the reference's f32 loop, so this is the reference's own f32-vs-FP64 distance "
+                             "(its operation order; the 1e-5 target holds against the f32 reference, bit-equal)"},
         "final_delta": float(deltas[-1]),
         "forms": forms,
     }
