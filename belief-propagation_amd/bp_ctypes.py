@@ -226,11 +226,12 @@ _lib = None
 
 
 def load(path: str | os.PathLike | None = None) -> C.CDLL:
-    """Loads libbp_amd.so once; raises if it has not been built."""
+    """Loads libbp_amd.so once (path, else $BP_AMD_LIB, else the in-tree
+    build); raises if it has not been built."""
     global _lib
     if _lib is not None:
         return _lib
-    p = Path(path) if path else LIB_PATH
+    p = Path(path) if path else Path(os.environ.get("BP_AMD_LIB") or LIB_PATH)
     if not p.exists():
         raise RuntimeError(f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
     lib = C.CDLL(str(p))
