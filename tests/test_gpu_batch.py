@@ -4,6 +4,7 @@ beliefs, messages and deltas equal running it alone through the plain path
 tuner can pick; and an instance equals the oracle."""
 from __future__ import annotations
 
+import ctypes as C
 import json
 
 import numpy as np
@@ -184,3 +185,105 @@ def test_device_node_order_does_not_change_batch_results(lib):
     for k, (a, c) in enumerate(zip(out[0][0], out[1][0])):
         assert all(same_bits(x, y) for x, y in zip(a, c)), f"instance {k}"
     assert np.allclose(out[0][1], out[1][1], rtol=1e-12, atol=1e-300)
+
+
+class _ResultShell:
+    """A Graph_t holding only what bp_batch_download writes: node and edge
+    counts, node_states and the two message arrays (zeroed, so entries
+    beyond each node's or edge's count stay 0, as B.states/B.messages mask
+    them), with the message pointers as create_graph sets them."""
+
+    def __init__(self, V: int, E: int):
+        self.states = np.zeros(max(V, 1) * 10, dtype=np.float32)
+        self.m_a = np.zeros(max(E, 1) * 10, dtype=np.float32)
+        self.m_b = np.zeros(max(E, 1) * 10, dtype=np.float32)
+        g = B.Graph()
+        g.total_num_vertices = g.current_num_vertices = V
+        g.total_num_edges = g.current_num_edges = E
+        f32p = C.POINTER(C.c_float)
+        g.node_states = self.states.ctypes.data_as(f32p)
+        g.edges_messages = self.m_a.ctypes.data_as(f32p)
+        g.last_edges_messages = self.m_b.ctypes.data_as(f32p)
+        self.g = g
+        self.ptr = C.pointer(g)
+        base = C.addressof(g)
+        g.current_edge_messages = C.cast(base + B.Graph.edges_messages.offset, C.POINTER(f32p))
+        g.previous_edge_messages = C.cast(base + B.Graph.last_edges_messages.offset, C.POINTER(f32p))
+        self.V, self.E = V, E
+
+    def arrays(self):
+        g = self.g
+        prev = np.ctypeslib.as_array(g.previous_edge_messages.contents, shape=(self.E * 10,))
+        cur = np.ctypeslib.as_array(g.current_edge_messages.contents, shape=(self.E * 10,))
+        return (self.states[: self.V * 10].reshape(self.V, 10), prev.reshape(self.E, 10), cur.reshape(self.E, 10))
+
+
+def _sha(a) -> str:
+    import hashlib
+
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+@pytest.mark.timeout(1200)
+def test_full_size_batch_matches_plain_runs_and_oracle(lib, orc, request):
+    """The batched workload at its stated size: 512 seeded DAGs (node counts
+    log-uniform in 200-50k, in-degree 0-4, states 2-8) x 8 evidence sets
+    (3% one-hot), init + 30 iterations in one batch, shape tuned.  Every
+    one of the 4096 instances -- beliefs, both message arrays and its FP64
+    deltas -- equals its plain run (bp_upload + bp_loopy_init +
+    bp_loopy_iterate, the network alone), and the first instance of the
+    largest network equals the oracle.  The plain runs are taken while the
+    batch is built (one pass over the networks) and kept as digests."""
+    import time
+
+    from workloads import _network_graphs, network_sizes
+
+    tr = request.config.pluginmanager.get_plugin("terminalreporter")
+
+    def progress(msg):
+        if tr is not None:
+            tr.write_line(f"full-size batch: {msg}")
+            tr._tw.flush()
+
+    count, sets, n = 512, 8, 30
+    sizes = network_sizes(count, 200, 50000, 5005)
+    big = int(np.argmax(sizes))
+    b = B.Batch()
+    plain = []
+    t0 = time.time()
+    for i, e, g in batched_networks(count=count, evidence_sets=sets):
+        b.add(g)
+        pd = plain_run(g, n)
+        st, mp, mc = copy_graph_results(lib, g)
+        plain.append((g.contents.current_num_vertices, g.contents.current_num_edges, _sha(st), _sha(mp), _sha(mc), pd))
+        lib.graph_destroy(g)
+        if e == sets - 1 and (i + 1) % 32 == 0:
+            progress(f"{i + 1} of {count} networks built and run alone ({time.time() - t0:.0f} s)")
+    b.upload(0)
+    b.tune(3)
+    progress(f"uploaded, shape {b.config()}")
+    b.init()
+    bd = b.iterate(n, want_deltas=True)
+    b.sync()
+    info = b.info()
+    assert info["instances"] == count * sets
+    for k, (V, E, s_st, s_mp, s_mc, pd) in enumerate(plain):
+        sh = _ResultShell(V, E)
+        b.download(k, sh.ptr)
+        st, mp, mc = sh.arrays()
+        assert _sha(st) == s_st, f"instance {k}: beliefs differ from its plain run"
+        assert _sha(mp) == s_mp and _sha(mc) == s_mc, f"instance {k}: messages differ from its plain run"
+        assert np.allclose(bd[:, k], pd, rtol=1e-12, atol=1e-300), f"instance {k}: deltas"
+        if k == big * sets:
+            big_states = st.copy()
+        if (k + 1) % 1024 == 0:
+            progress(f"{k + 1} of {count * sets} instances equal their plain runs")
+    (h,) = _network_graphs(big, sizes[big], 1, 0.03, 5005)
+    orc.init_previous_edge(h)
+    for _ in range(n):
+        orc.loopy_iteration(h, threads=8)
+    want = B.states(h)
+    assert same_bits(big_states, want), bits_report(big_states, want)
+    progress(f"largest network ({h.contents.current_num_vertices} nodes) equals the oracle")
+    lib.graph_destroy(h)
+    b.free()
