@@ -178,7 +178,7 @@ static int g_node_order = 1;
 extern "C" int bp_set_node_order(int mode)
 {
     int prev = g_node_order;
-    g_node_order = mode ? 1 : 0;
+    g_node_order = (mode == 0 || mode == 2) ? mode : 1;
     return prev;
 }
 
@@ -277,6 +277,90 @@ void bp_cm_order(Graph_t g, std::vector<unsigned> &order)
 #ifdef BP_ORDER_REVERSE
     std::reverse(order.begin(), order.end());
 #endif
+}
+
+/* Device node order, second step: nodes grouped by the XCDs that gather
+ * them.  The iteration kernel hands each of the 8 XCDs one contiguous range
+ * of chunks (xcd_block), so XCD x reads the combine buffers of its slots'
+ * sources into its own L2 in 128-byte lines.  In Cuthill-McKee order the
+ * sources an XCD needs from outside its range sit scattered between nodes
+ * it does not need, and the line floor (distinct lines each XCD touches)
+ * is about twice the row floor (distinct rows) on random DAGs.  Within
+ * each XCD's range, nodes are stably sorted by the set of XCDs that gather
+ * them (the ones only their own XCD gathers first), so rows gathered by the
+ * same XCDs share lines.  Every node keeps its in-edges in order, so
+ * results do not change.  The grouping is kept only when it lowers the
+ * line floor (on a lattice, whose gathers are local, it does not). */
+static uint64_t line_floor(Graph_t g, const std::vector<unsigned> &inv, const std::vector<unsigned> &xcd_of_pos,
+                           unsigned SP)
+{
+    unsigned V = g->current_num_vertices;
+    std::vector<unsigned> perm(V);
+    for (unsigned i = 0; i < V; ++i)
+        perm[inv[i]] = i;
+    size_t nlines = ((size_t)V * SP * 4 + 127) / 128 + 1;
+    std::vector<unsigned char> seen(nlines * 8, 0);
+    uint64_t lines = 0;
+    for (unsigned e = 0; e < g->current_num_edges; ++e) {
+        unsigned x = xcd_of_pos[perm[g->edges_dest_index[e]]];
+        size_t l = (size_t)perm[g->edges_src_index[e]] * SP * 4 / 128;
+        if (!seen[l * 8 + x]) {
+            seen[l * 8 + x] = 1;
+            ++lines;
+        }
+    }
+    return lines;
+}
+
+/* XCD of each device position under the iteration kernel's chunk split */
+static void xcd_ranges(Graph_t g, const std::vector<unsigned> &inv, unsigned cap, std::vector<unsigned> &xcd_of_pos,
+                       unsigned bounds[9])
+{
+    unsigned V = g->current_num_vertices;
+    std::vector<unsigned> in_off(V + 1, 0);
+    for (unsigned i = 0; i < V; ++i) {
+        unsigned h = inv[i];
+        in_off[i + 1] = in_off[i] + (bp_csr_end(g, g->dest_nodes_to_edges_node_list, h) -
+                                     g->dest_nodes_to_edges_node_list[h]);
+    }
+    std::vector<uint2> chunks;
+    bp_build_chunks(in_off, V, cap, chunks);
+    unsigned nch = (unsigned)chunks.size() - 1;
+    unsigned want = (nch + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
+    unsigned long long W = (unsigned long long)std::max(8u, (want + 7u) & ~7u) * WAVES_PER_BLOCK;
+    xcd_of_pos.assign(V, 0);
+    for (unsigned x = 0; x <= 8; ++x)
+        bounds[x] = chunks[(unsigned)((x * W / 8) * nch / W)].x;
+    for (unsigned x = 0; x < 8; ++x)
+        for (unsigned i = bounds[x]; i < bounds[x + 1]; ++i)
+            xcd_of_pos[i] = x;
+}
+
+static void xcd_group(Graph_t g, std::vector<unsigned> &inv, unsigned cap, unsigned SP)
+{
+    unsigned V = g->current_num_vertices;
+    if (V < 2)
+        return;
+    std::vector<unsigned> xpos;
+    unsigned bounds[9];
+    xcd_ranges(g, inv, cap, xpos, bounds);
+    std::vector<unsigned> perm(V);
+    for (unsigned i = 0; i < V; ++i)
+        perm[inv[i]] = i;
+    std::vector<unsigned char> mask(V, 0);
+    for (unsigned e = 0; e < g->current_num_edges; ++e)
+        mask[g->edges_src_index[e]] |= (unsigned char)(1u << xpos[perm[g->edges_dest_index[e]]]);
+    std::vector<unsigned> grouped(inv);
+    for (unsigned x = 0; x < 8; ++x) {
+        auto key = [&](unsigned v) { return mask[v] == (1u << x) ? -1 : (int)mask[v]; };
+        std::stable_sort(grouped.begin() + bounds[x], grouped.begin() + bounds[x + 1],
+                         [&](unsigned a, unsigned b) { return key(a) < key(b); });
+    }
+    std::vector<unsigned> xpos2;
+    unsigned bounds2[9];
+    xcd_ranges(g, grouped, cap, xpos2, bounds2);
+    if (line_floor(g, grouped, xpos2, SP) * 100 < line_floor(g, inv, xpos, SP) * 99)
+        inv.swap(grouped);
 }
 
 static int check_graph(Graph_t g)
@@ -516,8 +600,14 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
     unsigned V = g->current_num_vertices, E = g->current_num_edges;
     d->V = V;
     d->E = E;
+    unsigned S_all = 1;
+    for (unsigned v = 0; v < V; ++v)
+        S_all = std::max(S_all, g->node_num_vars[v]);
+    const unsigned cap = WAVE * ((S_all <= 4 && !(S_all >= 4 && BP_PLAIN_L > 1)) ? (unsigned)BP_SLOTS_SMALL : 1u);
     if (g_node_order && V > 1) {
         bp_cm_order(g, d->inv);
+        if (g_node_order == 1)
+            xcd_group(g, d->inv, cap, stride_of(S_all));
     } else {
         d->inv.resize(V);
         for (unsigned v = 0; v < V; ++v)
@@ -566,8 +656,7 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
         slot_xy[s] = (unsigned char)(g->edges_x_dim[e] | (g->edges_y_dim[e] << 4));
     }
     std::vector<uint2> chunks;
-    bp_build_chunks(in_off, V, WAVE * ((S <= 4 && !(S >= 4 && BP_PLAIN_L > 1)) ? (unsigned)BP_SLOTS_SMALL : 1u),
-                    chunks);
+    bp_build_chunks(in_off, V, cap, chunks);
     d->nchunks = (unsigned)chunks.size() - 1;
     /* one chunk per wave; a multiple of 8 blocks so each XCD gets one range */
     unsigned want = (d->nchunks + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;

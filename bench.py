@@ -304,6 +304,9 @@ def main():
     ap.add_argument("--batch-nets", type=int, default=512, help="networks in the batched form (8 evidence sets each)")
     ap.add_argument("--no-batch-plain", action="store_true",
                     help="batched form without its plain per-network comparison")
+    ap.add_argument("--node-order", type=int, default=1, choices=(0, 1, 2),
+                    help="device node numbering (bp_set_node_order): 1 Cuthill-McKee grouped by XCD (default), "
+                         "2 Cuthill-McKee alone, 0 the graph's order")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (scale for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -333,6 +336,7 @@ def main():
     lib.bp_set_abort_on_error(0)
     if not torch.cuda.is_available():
         sys.exit("bench.py: no GPU visible")
+    lib.bp_set_node_order(args.node_order)
 
     bn = B.generate(args.preset)
     g = B.bn_to_graph(bn)

@@ -28,6 +28,11 @@ from belief_network_wrapper import DifferentiableNetwork
 pytestmark = pytest.mark.gpu
 
 FLOOR = 1e-30
+# gradient tolerances against the float64 oracle per unrolled depth K
+# (measured on the 3000-node network: relative l2 1.8e-7, 1.1e-6, 2.7e-4 and
+# largest entry error over largest entry 8.0e-8, 9.1e-7, 2.8e-4 at K = 1, 6, 30)
+GRAD_REL = {1: 1e-6, 6: 5e-6, 30: 1e-3}
+GRAD_MAX = {1: 5e-7, 6: 5e-6, 30: 1e-3}
 
 
 def setup(preset="c5", **kw):
@@ -118,8 +123,14 @@ def test_gradient_matches_fp64_oracle(lib, K):
         return
     rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
     cos = g32 @ g64 / (np.linalg.norm(g32) * np.linalg.norm(g64))
-    print(f"K={K}: loss {float(loss):.6f} vs {float(l64):.6f}; gradient rel l2 {rel:.3e}, cosine {cos:.12f}")
-    assert rel < 2e-3
+    worst = np.abs(g32 - g64).max() / np.abs(g64).max()
+    print(f"K={K}: loss {float(loss.detach()):.6f} vs {float(l64.detach()):.6f}; gradient rel l2 {rel:.3e}, "
+          f"max entry error / max entry {worst:.3e}, cosine {cos:.12f}")
+    # a few times what the f32 recursion itself departs from float64: the
+    # error grows with the unrolled depth through the multiplicative belief
+    # accumulation
+    assert rel < GRAD_REL[K]
+    assert worst < GRAD_MAX[K]
     assert cos > 1 - 1e-6
     teardown(lib, bn, g, net)
 
@@ -228,6 +239,14 @@ def test_full_scale_c5(lib):
 
     l64 = loss64(th64)
     print(f"loss f32 {float(loss.detach()):.6f} fp64 {l64:.6f}", flush=True)
+    # every belief after K iterations against the float64 recursion
+    with torch.no_grad():
+        p64 = torch.softmax(th64[rows], dim=1).reshape(-1)[inv]
+        b64 = D.unrolled(st, st.tables(p64), st.initial(p64), K).numpy()
+        b32 = net.beliefs_from_log(theta.cuda(), K, 5).double().cpu().numpy()
+    fp64_maxabs = float(np.abs(b32 - b64).max())
+    print(f"c5 beliefs after K={K}: max |f32 - fp64| over all {net.V} nodes {fp64_maxabs:.3e}", flush=True)
+    assert fp64_maxabs < 3e-3  # measured 7.0e-4
     assert abs(float(loss.detach()) - l64) <= 1e-3 * max(1.0, abs(l64))
     rng = np.random.default_rng(1)
     for _ in range(2):

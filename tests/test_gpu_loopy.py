@@ -346,16 +346,21 @@ def test_convergence_drivers(lib, orc, variant):
     lib.graph_destroy(h)
 
 
-@pytest.mark.parametrize("case", ["c1", "lattice16", "bf_1000_2000_1"])
+@pytest.mark.parametrize("case", ["c1", "lattice16", "bf_1000_2000_1", "c2b_60k", "c2_60k"])
 def test_device_node_order_does_not_change_results(lib, case):
-    """Handles renumber nodes on the device (Cuthill-McKee by
-    default); the reference's order gives the same bits, and deltas that
-    differ only in the FP64 summation order."""
-    d = golden(case)
+    """Handles renumber nodes on the device (Cuthill-McKee, then grouped by
+    the XCDs that gather them, by default); Cuthill-McKee alone and the
+    reference's order give the same bits, and deltas that differ only in
+    the FP64 summation order."""
     out = []
-    for mode in (0, 1):
+    for mode in (0, 1, 2):
         prev = lib.bp_set_node_order(mode)
-        g = graph_for_case(d)
+        if case.endswith("_60k"):
+            bn = B.generate(case.split("_")[0], num_nodes=60000)
+            g = B.bn_to_graph(bn)
+            lib.bp_bn_free(bn)
+        else:
+            g = graph_for_case(golden(case))
         dev = B.Device(g)
         lib.bp_set_node_order(prev)
         dev.init()
@@ -364,8 +369,9 @@ def test_device_node_order_does_not_change_results(lib, case):
         out.append((B.states(g).copy(), B.messages(g, "prev").copy(), deltas))
         dev.free()
         lib.graph_destroy(g)
-    assert same_bits(out[0][0], out[1][0]) and same_bits(out[0][1], out[1][1])
-    assert np.allclose(out[0][2], out[1][2], rtol=1e-12, atol=0)
+    for k in (1, 2):
+        assert same_bits(out[0][0], out[k][0]) and same_bits(out[0][1], out[k][1]), f"mode {k}"
+        assert np.allclose(out[0][2], out[k][2], rtol=1e-12, atol=0), f"mode {k}"
 
 
 @pytest.mark.parametrize("edges", [0, 6])
