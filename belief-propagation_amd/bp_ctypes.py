@@ -458,7 +458,8 @@ class Device:
     def info(self) -> dict:
         a = (C.c_uint64 * 8)()
         self.lib.bp_dev_info(self.h, a)
-        return {"V": a[0], "E": a[1], "stride": a[2], "S": a[3], "chunks": a[4], "mixed": a[5], "tree_waves": a[6]}
+        return {"V": a[0], "E": a[1], "stride": a[2], "S": a[3], "chunks": a[4], "mixed": a[5], "tree_waves": a[6],
+                "shared_tables": a[7]}
 
     def init(self):
         check(self.lib.bp_loopy_init(self.h), "bp_loopy_init")

@@ -38,6 +38,13 @@ struct bp_dev {
     unsigned char *slot_xy = nullptr; /* [E] */
     float *J = nullptr;               /* [E][S*S]; mixed state counts: each slot's x*y floats, packed */
     unsigned *slot_toff = nullptr;    /* mixed state counts: [E] offset of each slot's table in J */
+    /* uniform state counts, transposed pairs shared (the loopy kernels
+     * only; tree BP and the adjoint read J): slot_pool[s] = index of the
+     * slot's table in Jp, bit 31 set when the slot reads it transposed */
+    bool shared = false;
+    float *Jp = nullptr;              /* [<= E][S*S] */
+    unsigned *slot_pool = nullptr;    /* [E] */
+    std::vector<unsigned> partner;    /* [E] by slot: the slot of the reverse edge, or ~0u */
     std::vector<unsigned> toff;       /* the same on the host */
     std::vector<unsigned> in_off_h;   /* [V+1] in_off on the host */
     size_t jfloats = 0;               /* floats in J */

@@ -106,7 +106,7 @@ __global__ void k_send_one(const float *J, const float *b, unsigned x, unsigned 
 
 static void free_dev_arrays(bp_dev *d)
 {
-    void *ptrs[] = {d->slot_toff, d->in_off, d->slot_src, d->slot_xy, d->J, d->nvars, d->states, d->buf[0], d->buf[1],
+    void *ptrs[] = {d->Jp, d->slot_pool, d->slot_toff, d->in_off, d->slot_src, d->slot_xy, d->J, d->nvars, d->states, d->buf[0], d->buf[1],
                     d->msg[0], d->msg[1], d->states0, d->chunk, d->partials, d->deltas, d->last, d->stop};
     for (void *p : ptrs)
         if (p)
@@ -524,6 +524,35 @@ static int upload_state(bp_dev *d, Graph_t g, bool tables)
         });
         HIP_TRY(hipMemcpyAsync(d->J, tmp, d->jfloats * sizeof(float), hipMemcpyHostToDevice, d->stream));
         HIP_TRY(hipStreamSynchronize(d->stream));
+        if (d->shared) {
+            /* tmp holds the per-slot tables [s][S*S]: each table once, in
+             * the order of the first slot that reads it; the later slot of
+             * a pair whose tables are exact transposes reads its partner's */
+            std::vector<unsigned> pool_of(d->E), tab(d->E);
+            std::vector<float> pool;
+            pool.reserve((size_t)d->E * S * S);
+            unsigned n = 0;
+            for (unsigned s = 0; s < d->E; ++s) {
+                unsigned f = d->partner[s];
+                bool reuse = false;
+                if (f != ~0u && f < s) {
+                    const float *A = tmp + (size_t)s * S * S, *Bt = tmp + (size_t)f * S * S;
+                    reuse = true;
+                    for (unsigned i = 0; i < S && reuse; ++i)
+                        for (unsigned j = 0; j < S && reuse; ++j)
+                            reuse = memcmp(&A[i * S + j], &Bt[j * S + i], 4) == 0;
+                }
+                if (reuse && !(tab[f] >> 31)) {
+                    tab[s] = pool_of[f] | 0x80000000u;
+                } else {
+                    pool_of[s] = n;
+                    tab[s] = n++;
+                    pool.insert(pool.end(), tmp + (size_t)s * S * S, tmp + (size_t)(s + 1) * S * S);
+                }
+            }
+            HIP_TRY(hipMemcpy(d->Jp, pool.data(), pool.size() * sizeof(float), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(d->slot_pool, tab.data(), (size_t)d->E * 4, hipMemcpyHostToDevice));
+        }
     }
     d->latest = 0;
     d->bcur = 0;
@@ -565,6 +594,54 @@ static double algorithmic_bytes(Graph_t g)
     for (unsigned v = 0; v < V; ++v)
         bytes += 12.0 * g->node_num_vars[v] + 4.0;
     return bytes;
+}
+
+/* Transposed pairs (a reverse edge whose table is its forward edge's
+ * transpose, xml-expression.c:402-406) can share one stored table: the
+ * later slot of the pair reads the earlier one's, transposed.  That only
+ * pays where the partner's table is still in the XCD's L2 when the later
+ * slot reads it, i.e. where partners sit close in slot order in the same
+ * XCD's chunk range (a lattice in Cuthill-McKee order; not a random DAG,
+ * whose partners are scattered and would each pull a 128-byte line for
+ * one table).  Sharing is switched on when at least 90% of the paired
+ * slots have their partner in the same XCD range within 65,536 slots. */
+static void plan_table_sharing(bp_dev *d, Graph_t g, const std::vector<uint2> &chunks)
+{
+    unsigned E = d->E;
+    d->partner.assign(E, ~0u);
+    std::vector<std::pair<unsigned long long, unsigned>> key(E);
+    for (unsigned e = 0; e < E; ++e)
+        key[e] = {((unsigned long long)g->edges_src_index[e] << 32) | g->edges_dest_index[e], e};
+    std::sort(key.begin(), key.end());
+    unsigned paired = 0, local = 0;
+    unsigned nch = (unsigned)chunks.size() - 1;
+    unsigned long long W = (unsigned long long)d->G * WAVES_PER_BLOCK;
+    unsigned bound[9];
+    for (unsigned x = 0; x <= 8; ++x)
+        bound[x] = chunks[(unsigned)((x * W / 8) * nch / W)].y;
+    auto xcd = [&](unsigned s) {
+        unsigned x = 0;
+        while (x < 7 && s >= bound[x + 1])
+            ++x;
+        return x;
+    };
+    for (unsigned e = 0; e < E; ++e) {
+        unsigned long long rev = ((unsigned long long)g->edges_dest_index[e] << 32) | g->edges_src_index[e];
+        auto it = std::lower_bound(key.begin(), key.end(), std::make_pair(rev, 0u));
+        if (it == key.end() || it->first != rev || g->edges_src_index[e] == g->edges_dest_index[e])
+            continue;
+        unsigned s = d->edge_slot[e], f = d->edge_slot[it->second];
+        if (d->partner[s] != ~0u || d->partner[f] != ~0u)
+            continue;
+        d->partner[s] = f;
+        d->partner[f] = s;
+        paired += 2;
+        if (xcd(s) == xcd(f) && (s > f ? s - f : f - s) <= 65536u)
+            local += 2;
+    }
+    d->shared = paired > 0 && (unsigned long long)local * 10 >= (unsigned long long)paired * 9;
+    if (!d->shared)
+        d->partner.clear();
 }
 
 static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
@@ -669,6 +746,8 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
     }
     d->G = G;
     d->NP = G * WAVES_PER_BLOCK;
+    if (plan && !d->mixed && S <= 4 && BP_SHARE_TABLES)
+        plan_table_sharing(d, g, chunks);
     /* delta partials of up to part_cap iterations are reduced per batch */
     d->part_cap = (unsigned)std::min<size_t>(256, std::max<size_t>(1, (64u << 20) / (8ull * d->NP)));
     d->alg_bytes = algorithmic_bytes(g);
@@ -698,6 +777,10 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
         }
     } else {
         d->jfloats = (size_t)E * S * S;
+    }
+    if (d->shared && (dmalloc(&d->Jp, (size_t)E * S * S) || dmalloc(&d->slot_pool, E))) {
+        bp_free(d);
+        return -1;
     }
     if (dmalloc(&d->in_off, V + 1) || dmalloc(&d->slot_src, E) || dmalloc(&d->slot_xy, E) ||
         dmalloc(&d->J, d->jfloats) || dmalloc(&d->nvars, V) || dmalloc(&d->states, V * SP) ||
@@ -772,7 +855,9 @@ LoopyArgs bp_make_args(bp_dev *d)
     a.nchunks = d->nchunks;
     a.partials = d->partials;
     a.stop = nullptr;
-    a.slot_tab = d->mixed ? d->slot_toff : nullptr;
+    a.slot_tab = d->mixed ? d->slot_toff : d->shared ? d->slot_pool : nullptr;
+    if (d->shared)
+        a.J = d->Jp;
     a.tabs = nullptr;
     a.chunk_partials = nullptr;
     a.queue = nullptr;
@@ -1360,6 +1445,7 @@ extern "C" int bp_dev_info(const bp_dev *d, uint64_t *info)
     info[4] = d->nchunks;
     info[5] = d->mixed ? 1 : 0;
     info[6] = bp_tree_plan_waves(d->tree);
+    info[7] = d->shared ? 1 : 0;
     return 0;
 }
 
@@ -1503,35 +1589,58 @@ extern "C" unsigned int loopy_propagate_until_hip_edge(Graph_t g, float converge
 }
 
 /* graph.c:426-448 on the GPU: one edge, tables in MAX_STATES stride */
+/* send_message's device buffer ([J | b | m]) and stream, one per device,
+ * created on first use and kept: a caller looping over edges pays one
+ * small copy each way per message, not an allocation and the synchronising
+ * hipFree of each call.  The lock makes concurrent calls take turns. */
+static constexpr int SEND_MAX_DEVICES = 64;
+static std::mutex g_send_mu;
+static struct {
+    float *buf;
+    hipStream_t stream;
+} g_send[SEND_MAX_DEVICES];
+
 extern "C" void send_message(float *states, unsigned int offset, unsigned int edge_index,
                              float *edge_joint_probabilities, float *edge_messages, unsigned int *edge_num_src,
                              unsigned int *edge_num_dest)
 {
+    constexpr size_t NJ = MAX_STATES * MAX_STATES;
     unsigned x = edge_num_src[edge_index], y = edge_num_dest[edge_index];
     if (x > MAX_STATES || y > MAX_STATES) {
         bp_fail("send_message: table %ux%u exceeds MAX_STATES", x, y);
         return;
     }
-    float J[MAX_STATES * MAX_STATES] = {0}, b[MAX_STATES] = {0}, m[MAX_STATES] = {0};
+    float in[NJ + MAX_STATES] = {0}, m[MAX_STATES] = {0};
     for (unsigned i = 0; i < x; ++i)
         for (unsigned j = 0; j < y; ++j)
-            J[i * MAX_STATES + j] = edge_joint_probabilities[(size_t)edge_index * MAX_STATES * MAX_STATES + i * MAX_STATES + j];
+            in[i * MAX_STATES + j] = edge_joint_probabilities[(size_t)edge_index * NJ + i * MAX_STATES + j];
     for (unsigned j = 0; j < y; ++j)
-        b[j] = states[offset + j];
-    float *dJ = nullptr, *db = nullptr, *dm = nullptr;
-    bool ok = hipMalloc((void **)&dJ, sizeof(J)) == hipSuccess && hipMalloc((void **)&db, sizeof(b)) == hipSuccess &&
-              hipMalloc((void **)&dm, sizeof(m)) == hipSuccess &&
-              hipMemcpy(dJ, J, sizeof(J), hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(db, b, sizeof(b), hipMemcpyHostToDevice) == hipSuccess;
-    if (ok) {
-        hipLaunchKernelGGL(k_send_one<MAX_STATES>, dim3(1), dim3(64), 0, 0, dJ, db, x, y, dm);
-        ok = hipGetLastError() == hipSuccess && hipMemcpy(m, dm, sizeof(m), hipMemcpyDeviceToHost) == hipSuccess;
-    }
-    if (dJ) (void)hipFree(dJ);
-    if (db) (void)hipFree(db);
-    if (dm) (void)hipFree(dm);
-    if (!ok) {
+        in[NJ + j] = states[offset + j];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SEND_MAX_DEVICES) {
         bp_fail("send_message: no usable HIP device");
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_send_mu);
+    auto &st = g_send[dev];
+    if (!st.buf) {
+        if (hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipMalloc((void **)&st.buf, (NJ + 2 * MAX_STATES) * sizeof(float)) != hipSuccess) {
+            st.buf = nullptr;
+            bp_fail("send_message: no usable HIP device");
+            return;
+        }
+    }
+    bool ok = hipMemcpyAsync(st.buf, in, sizeof(in), hipMemcpyHostToDevice, st.stream) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_send_one<MAX_STATES>, dim3(1), dim3(64), 0, st.stream, st.buf, st.buf + NJ, x, y,
+                           st.buf + NJ + MAX_STATES);
+        ok = hipGetLastError() == hipSuccess &&
+             hipMemcpyAsync(m, st.buf + NJ + MAX_STATES, sizeof(m), hipMemcpyDeviceToHost, st.stream) == hipSuccess &&
+             hipStreamSynchronize(st.stream) == hipSuccess;
+    }
+    if (!ok) {
+        bp_fail("send_message: the GPU send failed");
         return;
     }
     for (unsigned i = 0; i < x; ++i)

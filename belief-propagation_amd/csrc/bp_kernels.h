@@ -54,6 +54,11 @@
 #ifndef BP_XCD_REMAP
 #define BP_XCD_REMAP 1
 #endif
+/* uniform-state handles may share one stored table per transposed pair
+ * (bp_upload decides per graph); 0 keeps one table per slot */
+#ifndef BP_SHARE_TABLES
+#define BP_SHARE_TABLES 1
+#endif
 /* > 0: the iteration grid is capped at this many workgroups per CU and each
  * wave walks its share of chunks in a loop (0: one chunk per wave). */
 #ifndef BP_PERSIST
@@ -253,12 +258,32 @@ __device__ __forceinline__ void load_pooled_row(const float *__restrict__ T, uns
     }
 }
 
+/* A uniform-state table from the shared pool: entry slot_tab[s] of J,
+ * read transposed when bit 31 is set (the slot is the later one of a pair
+ * whose tables are exact transposes). */
+template <int S>
+__device__ __forceinline__ void load_shared_table(const LoopyArgs &a, unsigned s, float *J)
+{
+    unsigned t = a.slot_tab[s];
+    float T[S * S];
+    load_row<S * S>(a.J + (size_t)(t & 0x7fffffffu) * S * S, T);
+    const bool tr = (t >> 31) != 0;
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+            J[i * S + j] = tr ? T[j * S + i] : T[i * S + j];
+}
+
 /* The table of slot s as J[i*S + j] (zero beyond x rows and y columns). */
 template <int S, bool MIXED, bool PACKED>
 __device__ __forceinline__ void load_table(const LoopyArgs &a, unsigned s, unsigned xy, float *J)
 {
     if constexpr (!PACKED && !MIXED) {
-        load_row_nt<S * S>(a.J + (size_t)s * S * S, J);
+        if (BP_SHARE_TABLES && S <= 4 && a.slot_tab)
+            load_shared_table<S>(a, s, J);
+        else
+            load_row_nt<S * S>(a.J + (size_t)s * S * S, J);
     } else if constexpr (!PACKED) {
         unsigned x = xy & 15u, y = xy >> 4;
         const float *T = a.J + a.slot_tab[s];
@@ -288,6 +313,14 @@ template <int S, bool MIXED, bool PACKED>
 __device__ __forceinline__ void send_from_memory(const LoopyArgs &a, unsigned s, unsigned xy, const float *b, float *m)
 {
     unsigned x = MIXED ? (xy & 15u) : (unsigned)S, y = MIXED ? (xy >> 4) : (unsigned)S;
+    if constexpr (!PACKED && !MIXED && S <= 4) {
+        if (BP_SHARE_TABLES && a.slot_tab) {
+            float Js[S * S];
+            load_shared_table<S>(a, s, Js);
+            send<S, false>(Js, b, S, S, m);
+            return;
+        }
+    }
     const float *T = PACKED ? a.tabs + a.slot_tab[s] : MIXED ? a.J + a.slot_tab[s] : a.J + (size_t)s * S * S;
     float sum = 0.0f;
 #pragma unroll

@@ -371,11 +371,21 @@ int check_levels(Graph_t g)
             return -1;
         }
     }
-    for (unsigned k = V; k < 2 * V; ++k)
-        if (g->levels_to_nodes[k] >= V) {
-            bp_set_error("levels_to_nodes names node %u of %u", g->levels_to_nodes[k], V);
+    std::vector<unsigned char> seen(V, 0);
+    for (unsigned k = V; k < 2 * V; ++k) {
+        unsigned v = g->levels_to_nodes[k];
+        if (v >= V) {
+            bp_set_error("levels_to_nodes names node %u of %u", v, V);
             return -1;
         }
+        /* a node listed twice would break the sub-wave order the plan
+         * derives from each node's position in its level */
+        if (seen[v]) {
+            bp_set_error("levels_to_nodes lists node %u twice", v);
+            return -1;
+        }
+        seen[v] = 1;
+    }
     return 0;
 }
 

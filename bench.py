@@ -475,7 +475,8 @@ def main():
         "config": {"workload": f"{args.preset}: {PRESET_KIND.get(args.preset, 'seeded network')}, "
                                f"init + {ITERATIONS} loopy iterations",
                    "nodes": V, "arcs": arcs, "message_edges": E, "iterations": ITERATIONS,
-                   "state_stride": info["stride"], "delta": "fp64 per iteration"},
+                   "state_stride": info["stride"], "shared_pair_tables": bool(info["shared_tables"]),
+                   "delta": "fp64 per iteration"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_loopy<ITER>", "kernel_us": kernel_ms * 1e3,

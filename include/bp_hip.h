@@ -138,7 +138,8 @@ int bp_sync(bp_dev *d);
 
 /* info[0]=V, info[1]=E (message edges), info[2]=state stride,
  * info[3]=max state count, info[4]=chunks of the node kernel,
- * info[5]=1 if state counts are mixed, info[6]=tree sub-waves */
+ * info[5]=1 if state counts are mixed, info[6]=tree sub-waves,
+ * info[7]=1 if the loopy kernels read one stored table per transposed pair */
 int bp_dev_info(const bp_dev *d, uint64_t *info);
 
 /* Algorithmic HBM bytes of one loopy iteration of this graph, the bytes an

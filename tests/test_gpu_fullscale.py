@@ -32,6 +32,9 @@ def test_loopy_full_scale_digests(lib, case):
     S = d["sample_states"].shape[1]
     n = int(d["iters"])
     dev = B.Device(g)
+    # the lattice's transposed pairs sit close in slot order and share one
+    # stored table; the random DAG's are scattered and keep one per slot
+    assert dev.info()["shared_tables"] == (1 if case == "c3" else 0)
     dev.init()
     deltas = dev.iterate(n, want_deltas=True)
     dev.download(g)

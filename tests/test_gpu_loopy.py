@@ -237,7 +237,8 @@ def test_high_in_degree_matches_oracle(lib, orc, n_leaves, states):
 
 
 def test_send_message_dropin(lib, orc):
-    """send_message (graph.c:426-448) on one edge, run on the GPU."""
+    """send_message (graph.c:426-448), one call per edge over every edge of
+    the lattice, run on the GPU, each equal to the oracle's send."""
     import ctypes as C
 
     d = golden("lattice16")
@@ -245,7 +246,7 @@ def test_send_message_dropin(lib, orc):
     G = g.contents
     E = G.current_num_edges
     h = graph_for_case(d)
-    for e in (0, 7, E - 1):
+    for e in range(E):
         src = G.edges_src_index[e]
         lib.send_message(G.node_states, 10 * src, e, G.edges_joint_probabilities, G.edges_messages,
                          G.edges_x_dim, G.edges_y_dim)

@@ -110,3 +110,17 @@ def test_tree_mixed_states_hand_worked(lib, orc, path):
     assert same_bits(B.states(g), B.states(h))
     lib.graph_destroy(g)
     lib.graph_destroy(h)
+
+
+def test_duplicate_level_entries_are_rejected(lib):
+    """levels_to_nodes naming a node twice is refused (the per-level plan
+    derives each node's sub-wave from its position in the level)."""
+    d = golden("tree10k")
+    g = graph_for_case(d, levels=True)
+    G = g.contents
+    V = G.current_num_vertices
+    lv = B.view_u32(G.levels_to_nodes, 2 * V)
+    lv[V + 1] = lv[V]
+    lib.propagate_using_levels_start(g)
+    assert "twice" in B.last_error()
+    lib.graph_destroy(g)
