@@ -39,15 +39,20 @@ HBM_PEAK_GBS = 8000.0
 ITERATIONS = 20
 
 
-def read_traffic(name: str):
-    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.txt), or None."""
+def read_profile_value(name: str, key: str):
+    """A `key: value` line of a committed rocprofv3 summary (profiles/), or None."""
     p = ROOT / "profiles" / name
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":", 1)[1])
     return None
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/*.txt), or None."""
+    return read_profile_value(name, "traffic_bytes_per_launch")
 
 
 def progress(msg: str) -> None:
@@ -378,6 +383,8 @@ def main():
     alg = dev.algorithmic_bytes()
     achieved = alg / (kernel_ms * 1e-3) / 1e9
     traffic = read_traffic(f"loopy_iter_{args.preset}_traffic.txt")
+    # the same kernel's average duration in the committed rocprofv3 kernel trace
+    rocprof_us = read_profile_value(f"loopy_iter_{args.preset}_kernel_trace.md", "iteration_kernel_avg_us")
 
     # the GPU result of one full step, kept for the parity check below
     dev.restore()
@@ -480,7 +487,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_loopy<ITER>", "kernel_us": kernel_ms * 1e3,
-                     "algorithmic_bytes_per_launch": alg},
+                     "algorithmic_bytes_per_launch": alg,
+                     "frac_basis": "achieved and frac: HIP events at the kernel's own start and end "
+                                   "(hipExtLaunchKernelGGL), mean of 20 launches in this run; frac_rocprof: the "
+                                   "average duration in the committed rocprofv3 kernel trace of the same kernel",
+                     "rocprof_kernel_us": rocprof_us,
+                     "frac_rocprof": (alg / (rocprof_us * 1e-6) / 1e9 / HBM_PEAK_GBS) if rocprof_us else None},
         "cpu_baseline": cpu,
         "transfer": {"upload_ms": upload_ms, "download_ms": download_ms,
                      "note": "outside the timed region: bp_upload of the host graph (node order, chunk lists, "
