@@ -14,6 +14,15 @@
  * merges the instances of a network by node range, so a network's evidence
  * sets read their shared tables close together in time.
  *
+ * Rows are compact: a node's belief and combine-buffer rows hold its own
+ * state count rounded up to an even number of floats, a slot's message rows
+ * its x entries the same way, back to back in node (slot) order, instead of
+ * every row padded to the batch's largest state count (on 2-8 states, 5
+ * on average, 5.4 floats per row instead of 8).  slot_src holds the
+ * source's row offset; each chunk carries the row offsets of its first
+ * node and first slot, and the kernel finds the others by prefix sums of
+ * the widths across the wave.
+ *
  * The iteration kernel is k_loopy with packed tables and mixed state counts;
  * its arithmetic per node and slot is the plain path's, so every instance
  * gets the same bits as when it runs alone.
@@ -104,6 +113,10 @@ uint64_t hash_table(unsigned x, unsigned y, const float *T, bool transposed)
 struct Shape {
     unsigned R = 1;
     std::vector<uint2> chunks;
+    std::vector<uint2> base;          /* [nchunks] row offsets of each chunk's first node and first slot */
+    uint2 *d_base = nullptr;
+    std::vector<unsigned> wide;       /* chunks of one node with more than 64 R in-slots (k_loopy_wide) */
+    unsigned *d_wide = nullptr;
     std::vector<unsigned> inst_chunk; /* [instances+1] */
     std::vector<unsigned> order;      /* [nchunks] */
     uint2 *d_chunk = nullptr;
@@ -134,6 +147,9 @@ struct bp_batch {
     std::vector<unsigned> last_inv; /* the device order of the last instance added */
     std::vector<UniqueTable> uniq;
     std::unordered_map<uint64_t, unsigned> lookup; /* hash -> index in uniq */
+    /* compact rows: floats of node rows and of message rows before each
+     * instance ([instances+1]) */
+    std::vector<unsigned long long> node_row_off{0}, msg_row_off{0};
     double alg_bytes = 0.0;
     uint64_t distinct_floats = 0; /* a table and its transpose counted once */
     bool uploaded = false;
@@ -190,8 +206,8 @@ extern "C" void bp_batch_free(bp_batch *b)
         void *ps[] = {b->d_in_off, b->d_slot_src, b->d_slot_tab, b->d_queue, b->d_slot_xy, b->d_nvars, b->d_tabs,
                       b->d_states, b->d_states0, b->d_buf[0], b->d_buf[1], b->d_msg[0], b->d_msg[1],
                       b->d_partials, b->d_deltas, b->shapes[0].d_chunk, b->shapes[0].d_inst_chunk,
-                      b->shapes[0].d_order, b->shapes[1].d_chunk, b->shapes[1].d_inst_chunk,
-                      b->shapes[1].d_order};
+                      b->shapes[0].d_order, b->shapes[0].d_base, b->shapes[0].d_wide, b->shapes[1].d_chunk,
+                      b->shapes[1].d_inst_chunk, b->shapes[1].d_order, b->shapes[1].d_base, b->shapes[1].d_wide};
         for (void *p : ps)
             if (p)
                 (void)hipFree(p);
@@ -323,6 +339,21 @@ extern "C" int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index)
     std::vector<unsigned> perm(V);
     for (unsigned i = 0; i < V; ++i)
         perm[inv[i]] = i;
+    /* compact rows: each device node's row offset, then every slot's
+     * message widths */
+    const unsigned long long r0 = b->node_row_off.back(), m0 = b->msg_row_off.back();
+    std::vector<unsigned long long> row(V);
+    unsigned long long rt = r0, mt = m0;
+    for (unsigned i = 0; i < V; ++i) {
+        row[i] = rt;
+        rt += (g->node_num_vars[inv[i]] + 1u) & ~1u;
+    }
+    for (unsigned e = 0; e < E; ++e)
+        mt += (g->edges_x_dim[e] + 1u) & ~1u;
+    if (rt >= 0xffffffffull || mt >= 0xffffffffull) {
+        bp_set_error("bp_batch_add: the batch exceeds 2^32 floats of rows");
+        return -1;
+    }
     unsigned slots = 0;
     for (unsigned i = 0; i < V; ++i) {
         unsigned v = inv[i];
@@ -343,7 +374,7 @@ extern "C" int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index)
                 return -1;
             }
             unsigned u = intern_table(b, x, y, g->edges_joint_probabilities + (size_t)e * MAX_STATES * MAX_STATES);
-            b->slot_src.push_back(n0 + perm[g->edges_src_index[e]]);
+            b->slot_src.push_back((unsigned)row[perm[g->edges_src_index[e]]]);
             b->slot_tab.push_back(b->uniq[u].off);
             b->slot_xy.push_back((unsigned char)(x | (y << 4)));
             b->slot_edge.push_back(e);
@@ -353,6 +384,8 @@ extern "C" int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index)
     }
     b->node_off.push_back(n0 + V);
     b->slot_off.push_back(s0 + E);
+    b->node_row_off.push_back(rt);
+    b->msg_row_off.push_back(mt);
     b->inst_sig.push_back(sig);
     if (index)
         *index = (unsigned)b->node_off.size() - 2;
@@ -389,6 +422,24 @@ static void build_shape(bp_batch *b, Shape &sh, unsigned R)
     }
     sh.nchunks = (unsigned)sh.chunks.size();
     sh.chunks.push_back(make_uint2(b->V, b->E));
+    /* row offsets of each chunk's first node and first slot: prefix sums
+     * of the even-rounded widths, walked along with the chunks */
+    sh.base.resize(sh.nchunks);
+    {
+        unsigned long long nr = 0, mr = 0;
+        unsigned v = 0, s = 0;
+        for (unsigned c = 0; c < sh.nchunks; ++c) {
+            for (; v < sh.chunks[c].x; ++v)
+                nr += (b->nvars[v] + 1u) & ~1u;
+            for (; s < sh.chunks[c].y; ++s)
+                mr += (b->edge_x[s] + 1u) & ~1u;
+            sh.base[c] = make_uint2((unsigned)nr, (unsigned)mr);
+        }
+    }
+    sh.wide.clear();
+    for (unsigned c = 0; c < sh.nchunks; ++c)
+        if (sh.chunks[c + 1].y - sh.chunks[c].y > cap)
+            sh.wide.push_back(c);
     sh.order.clear();
     sh.order.reserve(sh.nchunks);
     std::vector<std::pair<uint64_t, unsigned>> keys;
@@ -456,11 +507,18 @@ extern "C" int bp_batch_upload(bp_batch *b, int device)
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     b->uploaded = true; /* from here on bp_batch_free releases device memory */
 
-    size_t VS = (size_t)b->V * b->SP, ES = (size_t)b->E * b->SP;
+    /* compact rows; S floats of slack past the last row, which the kernels'
+     * unpredicated row loads may read */
+    size_t VS = b->node_row_off.back() + b->S + 1, ES = b->msg_row_off.back() + b->S + 1;
     std::vector<float> st(VS, 0.0f);
-    for (size_t v = 0; v < b->V; ++v)
-        for (unsigned k = 0; k < b->nvars[v]; ++k)
-            st[v * b->SP + k] = b->st_stage[v * MAX_STATES + k];
+    {
+        size_t r = 0;
+        for (size_t v = 0; v < b->V; ++v) {
+            for (unsigned k = 0; k < b->nvars[v]; ++k)
+                st[r + k] = b->st_stage[v * MAX_STATES + k];
+            r += (b->nvars[v] + 1u) & ~1u;
+        }
+    }
     std::vector<unsigned> in_off(b->in_off);
     in_off.push_back(b->E);
     unsigned maxchunks = std::max(b->shapes[0].nchunks, b->shapes[1].nchunks);
@@ -477,7 +535,8 @@ extern "C" int bp_batch_upload(bp_batch *b, int device)
             continue;
         if (up(&sh.d_chunk, sh.chunks.data(), sh.chunks.size()) ||
             up(&sh.d_inst_chunk, sh.inst_chunk.data(), sh.inst_chunk.size()) ||
-            up(&sh.d_order, sh.order.data(), sh.order.size()))
+            up(&sh.d_order, sh.order.data(), sh.order.size()) || up(&sh.d_base, sh.base.data(), sh.base.size()) ||
+            up(&sh.d_wide, sh.wide.data(), sh.wide.size()))
             return -1;
     }
     HIP_TRY(hipMemcpy(b->d_states, b->d_states0, VS * sizeof(float), hipMemcpyDeviceToDevice));
@@ -555,6 +614,7 @@ static LoopyArgs batch_args(bp_batch *b)
     a.msg_in = b->d_msg[b->latest];
     a.msg_out = b->d_msg[b->latest ^ 1];
     a.chunk = sh.d_chunk;
+    a.chunk_base = sh.d_base;
     a.order = sh.d_order;
     a.nchunks = sh.nchunks;
     a.partials = nullptr;
@@ -578,44 +638,50 @@ static unsigned batch_grid(const bp_batch *b)
 }
 
 template <int S, int R, int L>
-static void launch_batch_srl(int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
+static void launch_batch_srl(int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev,
+                             const Shape &sh)
 {
     dim3 grid(G), block(BLOCK);
+    unsigned nw = (unsigned)sh.wide.size(), gw = (nw + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
     if (mode == ITER) {
         if (ev)
             hipExtLaunchKernelGGL((k_loopy<S, true, ITER, R, true, L>), grid, block, 0, st, ev[0], ev[1], 0, a);
         else
             hipLaunchKernelGGL((k_loopy<S, true, ITER, R, true, L>), grid, block, 0, st, a);
+        if (nw)
+            hipLaunchKernelGGL((k_loopy_wide<S, true, ITER>), dim3(gw), block, 0, st, a, sh.d_wide, nw);
     } else {
         hipLaunchKernelGGL((k_loopy<S, true, INIT, R, true, L>), grid, block, 0, st, a);
+        if (nw)
+            hipLaunchKernelGGL((k_loopy_wide<S, true, INIT>), dim3(gw), block, 0, st, a, sh.d_wide, nw);
     }
 }
 
 template <int S>
 static void launch_batch_s(unsigned R, unsigned L, int mode, const LoopyArgs &a, unsigned G, hipStream_t st,
-                           const hipEvent_t *ev)
+                           const hipEvent_t *ev, const Shape &sh)
 {
     if constexpr (S <= 6) {
         if (R == 2) {
-            launch_batch_srl<S, 2, 1>(mode, a, G, st, ev);
+            launch_batch_srl<S, 2, 1>(mode, a, G, st, ev, sh);
             return;
         }
     }
     if constexpr (S >= 4) {
         if (L == 2) {
-            launch_batch_srl<S, 1, 2>(mode, a, G, st, ev);
+            launch_batch_srl<S, 1, 2>(mode, a, G, st, ev, sh);
             return;
         }
         if (L == 4) {
-            launch_batch_srl<S, 1, 4>(mode, a, G, st, ev);
+            launch_batch_srl<S, 1, 4>(mode, a, G, st, ev, sh);
             return;
         }
         if (L == 8) {
-            launch_batch_srl<S, 1, 8>(mode, a, G, st, ev);
+            launch_batch_srl<S, 1, 8>(mode, a, G, st, ev, sh);
             return;
         }
     }
-    launch_batch_srl<S, 1, 1>(mode, a, G, st, ev);
+    launch_batch_srl<S, 1, 1>(mode, a, G, st, ev, sh);
 }
 
 static void launch_batch(bp_batch *b, int mode, const LoopyArgs &a, hipStream_t st, const hipEvent_t *ev = nullptr)
@@ -625,17 +691,18 @@ static void launch_batch(bp_batch *b, int mode, const LoopyArgs &a, hipStream_t 
     if (b->qblock)
         hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(64), 0, st, b->d_queue);
     unsigned G = batch_grid(b);
+    const Shape &sh = cur_shape(b);
     switch (b->S) {
-    case 1: launch_batch_s<1>(b->R, b->L, mode, a, G, st, ev); break;
-    case 2: launch_batch_s<2>(b->R, b->L, mode, a, G, st, ev); break;
-    case 3: launch_batch_s<3>(b->R, b->L, mode, a, G, st, ev); break;
-    case 4: launch_batch_s<4>(b->R, b->L, mode, a, G, st, ev); break;
-    case 5: launch_batch_s<5>(b->R, b->L, mode, a, G, st, ev); break;
-    case 6: launch_batch_s<6>(b->R, b->L, mode, a, G, st, ev); break;
-    case 7: launch_batch_s<7>(b->R, b->L, mode, a, G, st, ev); break;
-    case 8: launch_batch_s<8>(b->R, b->L, mode, a, G, st, ev); break;
-    case 9: launch_batch_s<9>(b->R, b->L, mode, a, G, st, ev); break;
-    default: launch_batch_s<10>(b->R, b->L, mode, a, G, st, ev); break;
+    case 1: launch_batch_s<1>(b->R, b->L, mode, a, G, st, ev, sh); break;
+    case 2: launch_batch_s<2>(b->R, b->L, mode, a, G, st, ev, sh); break;
+    case 3: launch_batch_s<3>(b->R, b->L, mode, a, G, st, ev, sh); break;
+    case 4: launch_batch_s<4>(b->R, b->L, mode, a, G, st, ev, sh); break;
+    case 5: launch_batch_s<5>(b->R, b->L, mode, a, G, st, ev, sh); break;
+    case 6: launch_batch_s<6>(b->R, b->L, mode, a, G, st, ev, sh); break;
+    case 7: launch_batch_s<7>(b->R, b->L, mode, a, G, st, ev, sh); break;
+    case 8: launch_batch_s<8>(b->R, b->L, mode, a, G, st, ev, sh); break;
+    case 9: launch_batch_s<9>(b->R, b->L, mode, a, G, st, ev, sh); break;
+    default: launch_batch_s<10>(b->R, b->L, mode, a, G, st, ev, sh); break;
     }
 }
 
@@ -647,7 +714,7 @@ extern "C" int bp_batch_init(bp_batch *b)
         return -1;
     }
     HIP_TRY(hipSetDevice(b->device));
-    size_t VS = (size_t)b->V * b->SP;
+    size_t VS = b->node_row_off.back();
     if (VS)
         hipLaunchKernelGGL(k_copy_batch, dim3(nblocks(VS)), dim3(256), 0, b->stream, b->d_states0, b->d_states, VS);
     b->latest = 0;
@@ -763,18 +830,21 @@ extern "C" int bp_batch_download(bp_batch *b, unsigned int index, Graph_t g)
         return -1;
     }
     HIP_TRY(hipSetDevice(b->device));
-    unsigned SP = b->SP;
-    std::vector<float> st((size_t)V * SP), m0((size_t)E * SP), m1((size_t)E * SP);
-    HIP_TRY(hipMemcpyAsync(st.data(), b->d_states + (size_t)n0 * SP, st.size() * 4, hipMemcpyDeviceToHost, b->stream));
-    HIP_TRY(hipMemcpyAsync(m0.data(), b->d_msg[b->latest] + (size_t)s0 * SP, m0.size() * 4, hipMemcpyDeviceToHost,
-                           b->stream));
-    HIP_TRY(hipMemcpyAsync(m1.data(), b->d_msg[b->latest ^ 1] + (size_t)s0 * SP, m1.size() * 4,
-                           hipMemcpyDeviceToHost, b->stream));
+    size_t r0 = b->node_row_off[index], rn = b->node_row_off[index + 1] - r0;
+    size_t q0 = b->msg_row_off[index], qn = b->msg_row_off[index + 1] - q0;
+    std::vector<float> st(rn), m0(qn), m1(qn);
+    if (rn)
+        HIP_TRY(hipMemcpyAsync(st.data(), b->d_states + r0, rn * 4, hipMemcpyDeviceToHost, b->stream));
+    if (qn) {
+        HIP_TRY(hipMemcpyAsync(m0.data(), b->d_msg[b->latest] + q0, qn * 4, hipMemcpyDeviceToHost, b->stream));
+        HIP_TRY(hipMemcpyAsync(m1.data(), b->d_msg[b->latest ^ 1] + q0, qn * 4, hipMemcpyDeviceToHost, b->stream));
+    }
     HIP_TRY(hipStreamSynchronize(b->stream));
-    for (unsigned i = 0; i < V; ++i) {
-        unsigned v = b->node_inv[n0 + i];
-        for (unsigned k = 0; k < b->nvars[n0 + i]; ++k)
-            g->node_states[(size_t)v * MAX_STATES + k] = st[(size_t)i * SP + k];
+    for (size_t i = 0, r = 0; i < V; ++i) {
+        unsigned v = b->node_inv[n0 + i], n = b->nvars[n0 + i];
+        for (unsigned k = 0; k < n; ++k)
+            g->node_states[(size_t)v * MAX_STATES + k] = st[r + k];
+        r += (n + 1u) & ~1u;
     }
     if (b->swaps & 1u) {
         float **t = g->previous_edge_messages;
@@ -782,12 +852,13 @@ extern "C" int bp_batch_download(bp_batch *b, unsigned int index, Graph_t g)
         g->current_edge_messages = t;
     }
     float *prev = *g->previous_edge_messages, *cur = *g->current_edge_messages;
-    for (unsigned k = 0; k < E; ++k) {
+    for (size_t k = 0, r = 0; k < E; ++k) {
         unsigned e = b->slot_edge[s0 + k], x = b->edge_x[s0 + k];
         for (unsigned i = 0; i < x; ++i) {
-            prev[(size_t)e * MAX_STATES + i] = m0[(size_t)k * SP + i];
-            cur[(size_t)e * MAX_STATES + i] = m1[(size_t)k * SP + i];
+            prev[(size_t)e * MAX_STATES + i] = m0[r + i];
+            cur[(size_t)e * MAX_STATES + i] = m1[r + i];
         }
+        r += (x + 1u) & ~1u;
     }
     return 0;
 }

@@ -851,6 +851,7 @@ LoopyArgs bp_make_args(bp_dev *d)
     a.msg_in = d->msg[d->latest];
     a.msg_out = d->msg[d->latest ^ 1];
     a.chunk = d->chunk;
+    a.chunk_base = nullptr;
     a.order = nullptr;
     a.nchunks = d->nchunks;
     a.partials = d->partials;

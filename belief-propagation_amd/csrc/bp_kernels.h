@@ -111,6 +111,7 @@ struct LoopyArgs {
     const float *msg_in;         /* [E][SP] old messages (ITER) / messages (PREP) */
     float *msg_out;              /* [E][SP] new messages (ITER, INIT) */
     const uint2 *chunk;          /* [nchunks+1] (first node, first slot) of each chunk */
+    const uint2 *chunk_base;     /* PACKED: [nchunks] row offsets (floats) of the chunk's first node and first slot */
     const unsigned *order;       /* non-null: [nchunks] the p-th chunk taken is chunk order[p] */
     unsigned nchunks;
     double *partials;            /* [G * waves per block] this launch's per-wave delta partials (ITER) */
@@ -354,45 +355,124 @@ __device__ __forceinline__ void send_from_memory(const LoopyArgs &a, unsigned s,
         m[i] = m[i] / sum;
 }
 
-/* The message of slot s into m[0..S) (zero beyond x), without side effects. */
+/* ---- compact rows (PACKED: the batch layout) ------------------------------
+ * A node's belief and combine-buffer rows hold its n entries rounded up to
+ * an even count (zero-padded), and a slot's message rows its x entries the
+ * same way, back to back in node (slot) order: rows start 8-byte aligned
+ * and no row carries the padding to the largest state count.  A chunk's
+ * first node and first slot rows are in chunk_base; the others follow by
+ * wave-wide prefix sums of the widths.  slot_src holds the row offset of
+ * the slot's source instead of its node number. */
+__device__ __forceinline__ unsigned even_width(unsigned n) { return (n + 1u) & ~1u; }
+
+/* exclusive prefix sum of v across the 64 lanes */
+__device__ __forceinline__ unsigned wave_excl_scan(unsigned v, unsigned lane)
+{
+    unsigned x = v;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        unsigned y = __shfl_up(x, off, WAVE);
+        if (lane >= (unsigned)off)
+            x += y;
+    }
+    return x - v;
+}
+
+/* the sum of v across the 64 lanes, given its exclusive prefix sum */
+__device__ __forceinline__ unsigned wave_total(unsigned excl, unsigned v) { return __shfl(excl + v, WAVE - 1, WAVE); }
+
+/* The first S floats at p (a compact row and whatever follows it: the
+ * buffers carry S floats of slack past their last row), 8-byte loads,
+ * unpredicated.  out[k] for k >= the row's count is not the row's: callers
+ * that read those entries use load_crow_z. */
+template <int S>
+__device__ __forceinline__ void load_crow(const float *__restrict__ p, float *out)
+{
+#pragma unroll
+    for (int q = 0; q < (S + 1) / 2; ++q) {
+        float2 v = *reinterpret_cast<const float2 *>(p + 2 * q);
+        out[2 * q] = v.x;
+        if (2 * q + 1 < S)
+            out[2 * q + 1] = v.y;
+    }
+}
+
+/* out[k] = row[k] for k < n, 0 beyond (a compact row of n entries read
+ * zero-extended, as a gathered source row must be) */
+template <int S>
+__device__ __forceinline__ void load_crow_z(const float *__restrict__ p, unsigned n, float *out)
+{
+    load_crow<S>(p, out);
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        out[k] = ((unsigned)k < n) ? out[k] : 0.0f;
+}
+
+/* the compact row of n entries v[0..n), zero-padded to an even count */
+template <int S>
+__device__ __forceinline__ void store_crow(float *__restrict__ p, unsigned n, const float *v)
+{
+#pragma unroll
+    for (int q = 0; q < (S + 1) / 2; ++q) {
+        if ((unsigned)(2 * q) < n) {
+            float hi = (2 * q + 1 < S && (unsigned)(2 * q + 1) < n) ? v[2 * q + 1] : 0.0f;
+            *reinterpret_cast<float2 *>(p + 2 * q) = make_float2(v[2 * q], hi);
+        }
+    }
+}
+
+/* The message of slot s into m[0..S) (zero beyond x), without side effects.
+ * mrow: the slot's message row offset (PACKED). */
 template <int S, bool MIXED, int MODE, bool PACKED>
-__device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, float *m, unsigned &x)
+__device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, float *m, unsigned &x,
+                                                unsigned mrow = 0)
 {
     constexpr int SP = Stride<S>::value;
     x = S;
-    if (MODE == PREP) {
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-            m[i] = a.msg_in[(size_t)s * SP + i];
-        return;
-    }
-    unsigned u = a.slot_src[s];
     unsigned y = S;
     if (MIXED) {
         unsigned xy = a.slot_xy[s];
         x = xy & 15u;
         y = xy >> 4;
     }
+    if (MODE == PREP) {
+        if constexpr (PACKED) {
+            load_crow_z<S>(a.msg_in + mrow, x, m);
+        } else {
+#pragma unroll
+            for (int i = 0; i < S; ++i)
+                m[i] = a.msg_in[(size_t)s * SP + i];
+        }
+        return;
+    }
+    unsigned u = a.slot_src[s];
     const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
     float b[S];
+    if constexpr (PACKED) {
 #pragma unroll
-    for (int j = 0; j < S; ++j)
-        b[j] = src[(size_t)u * SP + j];
+        for (int j = 0; j < S; ++j)
+            b[j] = ((unsigned)j < x) ? src[u + j] : 0.0f; /* the source's row: x = its state count */
+    } else {
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+            b[j] = src[(size_t)u * SP + j];
+    }
     send_from_memory<S, MIXED, PACKED>(a, s, x | (y << 4), b, m);
 }
 
 /* Slot phase: the message of slot s, written to msg_out (ITER, INIT, EDGE)
  * with its |new - old| added to dacc (ITER, EDGE). */
 template <int S, bool MIXED, int MODE, bool PACKED>
-__device__ __forceinline__ void slot_message(const LoopyArgs &a, unsigned s, float *m, double &dacc)
+__device__ __forceinline__ void slot_message(const LoopyArgs &a, unsigned s, float *m, double &dacc,
+                                             unsigned mrow = 0)
 {
     constexpr int SP = Stride<S>::value;
     unsigned x;
-    compute_message<S, MIXED, MODE, PACKED>(a, s, m, x);
+    compute_message<S, MIXED, MODE, PACKED>(a, s, m, x, mrow);
     if (MODE == PREP || MODE == FWD || MODE == FINIT)
         return;
     if (MODE == ITER || MODE == EDGE) {
-        const float *old = a.msg_in + (size_t)s * SP;
+        const float *old = PACKED ? a.msg_in + mrow : a.msg_in + (size_t)s * SP;
 #pragma unroll
         for (int i = 0; i < S; ++i) {
             if (!MIXED || (unsigned)i < x) {
@@ -403,10 +483,18 @@ __device__ __forceinline__ void slot_message(const LoopyArgs &a, unsigned s, flo
             }
         }
     }
-    float *dst = a.msg_out + (size_t)s * SP;
+    if constexpr (PACKED) {
+        float *dst = a.msg_out + mrow;
 #pragma unroll
-    for (int i = 0; i < SP; ++i)
-        dst[i] = (i < S) ? m[i] : 0.0f;
+        for (int i = 0; i < S; ++i)
+            if ((unsigned)i < ((x + 1u) & ~1u))
+                dst[i] = ((unsigned)i < x) ? m[i] : 0.0f;
+    } else {
+        float *dst = a.msg_out + (size_t)s * SP;
+#pragma unroll
+        for (int i = 0; i < SP; ++i)
+            dst[i] = (i < S) ? m[i] : 0.0f;
+    }
 }
 
 /* buf[k] = buf[k] * m[k] unless m[k] is NaN (graph.c:450-459) */
@@ -476,35 +564,49 @@ __device__ __forceinline__ void normalise(float *v, unsigned n)
             v[k] = v[k] / sum;
 }
 
-template <int S, bool MIXED, int MODE>
-__device__ __forceinline__ void write_node(const LoopyArgs &a, unsigned v, const float *st, const float *b, unsigned n)
+/* row: the node's row offset in floats (v * SP, or its compact row when
+ * PACKED) */
+template <int S, bool MIXED, int MODE, bool PACKED = false>
+__device__ __forceinline__ void write_node(const LoopyArgs &a, size_t row, const float *st, const float *b, unsigned n)
 {
     constexpr int SP = Stride<S>::value;
+    if constexpr (PACKED) {
+        if (MODE == EDGE) {
+            store_crow<S>(a.states_out + row, n, b);
+            return;
+        }
+        if (MODE == FWD)
+            store_crow<S>(a.states_out + row, n, st);
+        if (MODE == ITER)
+            store_crow<S>(a.states + row, n, st);
+        store_crow<S>(a.buf_out + row, n, b);
+        return;
+    }
     if (MODE == EDGE) { /* the new beliefs are the product */
 #pragma unroll
         for (int k = 0; k < SP; ++k)
-            a.states_out[(size_t)v * SP + k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
+            a.states_out[row + k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
         return;
     }
     if (MODE == FWD) {
 #pragma unroll
         for (int k = 0; k < SP; ++k)
-            a.states_out[(size_t)v * SP + k] = (k < S && (!MIXED || (unsigned)k < n)) ? st[k] : 0.0f;
+            a.states_out[row + k] = (k < S && (!MIXED || (unsigned)k < n)) ? st[k] : 0.0f;
     }
     /* whole padded rows (zeros beyond the node's state count): a partly
      * written row costs more than the zeros (measured on mixed networks) */
     if (MODE == ITER) {
-        float row[SP];
+        float r[SP];
 #pragma unroll
         for (int k = 0; k < SP; ++k)
-            row[k] = (k < S && (!MIXED || (unsigned)k < n)) ? st[k] : 0.0f;
-        store_row_nt<SP>(a.states + (size_t)v * SP, row);
+            r[k] = (k < S && (!MIXED || (unsigned)k < n)) ? st[k] : 0.0f;
+        store_row_nt<SP>(a.states + row, r);
     }
-    float row[SP];
+    float r[SP];
 #pragma unroll
     for (int k = 0; k < SP; ++k)
-        row[k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
-    store_row_nt<SP>(a.buf_out + (size_t)v * SP, row);
+        r[k] = (k < S && (!MIXED || (unsigned)k < n)) ? b[k] : 0.0f;
+    store_row_nt<SP>(a.buf_out + row, r);
 }
 
 /* Row loads: 8- and 16-byte vector loads where the row size allows. */
@@ -538,8 +640,8 @@ __device__ __forceinline__ void load_row(const float *__restrict__ p, float *out
  * (messages at lds[k * W + q]): the belief update of graph.c:985-1024
  * (ITER, FWD), the combine buffer of graph.c:851-882 for the next
  * iteration, or the edge schedule's product (EDGE). */
-template <int S, bool MIXED, int MODE>
-__device__ __forceinline__ void node_update(const LoopyArgs &a, unsigned v, unsigned q0, unsigned q1, float *st,
+template <int S, bool MIXED, int MODE, bool PACKED = false>
+__device__ __forceinline__ void node_update(const LoopyArgs &a, size_t row, unsigned q0, unsigned q1, float *st,
                                             unsigned n, const float *lds, int W)
 {
     float m[S], b[S];
@@ -573,7 +675,7 @@ __device__ __forceinline__ void node_update(const LoopyArgs &a, unsigned v, unsi
     }
     if (MODE == EDGE)
         normalise_positive<S, MIXED>(b, n);
-    write_node<S, MIXED, MODE>(a, v, st, b, n);
+    write_node<S, MIXED, MODE, PACKED>(a, row, st, b, n);
 }
 
 /* The first S floats of a row of SP: one 16-byte load when a 3-state row
@@ -607,7 +709,7 @@ __device__ __forceinline__ void load_rowp_nt(const float *__restrict__ p, float 
  * R slots' and R nodes' worth of reads in flight across one round trip. */
 template <int S, bool MIXED, int MODE, int R, bool PACKED>
 __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se,
-                                             float *lds, unsigned lane, double &dacc)
+                                             float *lds, unsigned lane, double &dacc, uint2 base)
 {
     constexpr int SP = Stride<S>::value;
     constexpr int W = WAVE * R;
@@ -628,14 +730,18 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
         unsigned s = sb + r * WAVE + lane;
         if (s >= se)
             s = s_last;
+        xy[r] = (unsigned)(S | (S << 4));
         if (any_slots) {
             if (MODE != PREP) {
                 u[r] = BP_NT_LOADS ? __builtin_nontemporal_load(a.slot_src + s) : a.slot_src[s];
-                xy[r] = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
+                if constexpr (MIXED)
+                    xy[r] = a.slot_xy[s];
                 if constexpr (HOLD)
                     load_table<S, MIXED, PACKED>(a, s, xy[r], J[r]);
+            } else if constexpr (PACKED) {
+                xy[r] = a.slot_xy[s];
             }
-            if (MODE == ITER || MODE == EDGE || MODE == PREP)
+            if (!PACKED && (MODE == ITER || MODE == EDGE || MODE == PREP))
                 load_rowp_nt<S, SP>(a.msg_in + (size_t)s * SP, old[r]);
         }
         unsigned v = nb + r * WAVE + lane;
@@ -644,18 +750,43 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
         q0[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v) : a.in_off[v]) - sb;
         q1[r] = (BP_NT_LOADS ? __builtin_nontemporal_load(a.in_off + v + 1) : a.in_off[v + 1]) - sb;
         /* INIT and EDGE gather beliefs: keep them cached there */
-        if (MODE == ITER || MODE == PREP || MODE == FWD)
-            load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st[r]);
-        else
-            load_rowp<S, SP>(a.states + (size_t)v * SP, st[r]);
+        if constexpr (!PACKED) {
+            if (MODE == ITER || MODE == PREP || MODE == FWD)
+                load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st[r]);
+            else
+                load_rowp<S, SP>(a.states + (size_t)v * SP, st[r]);
+        }
         nn[r] = MIXED ? a.nvars[v] : S;
+    }
+    /* compact rows: each lane's message and node row offsets */
+    unsigned mrow[R], nrow[R];
+    if constexpr (PACKED) {
+        unsigned mb = base.y, vb = base.x;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            unsigned s = sb + r * WAVE + lane, v = nb + r * WAVE + lane;
+            unsigned xs = (any_slots && s < se) ? (xy[r] & 15u) : 0u, nv = (v < ne) ? nn[r] : 0u;
+            unsigned w = even_width(xs), ex = wave_excl_scan(w, lane);
+            mrow[r] = mb + ex;
+            mb += wave_total(ex, w);
+            unsigned wn = even_width(nv), exn = wave_excl_scan(wn, lane);
+            nrow[r] = vb + exn;
+            vb += wave_total(exn, wn);
+            if (MODE == ITER || MODE == EDGE || MODE == PREP)
+                load_crow<S>(a.msg_in + mrow[r], old[r]);
+            load_crow<S>(a.states + nrow[r], st[r]);
+        }
     }
     float bsrc[R][S];
     if (MODE != PREP && any_slots) {
         const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-            load_rowp<S, SP>(src + (size_t)u[r] * SP, bsrc[r]);
+        for (int r = 0; r < R; ++r) {
+            if constexpr (PACKED)
+                load_crow_z<S>(src + u[r], xy[r] & 15u, bsrc[r]); /* the source's row: x entries */
+            else
+                load_rowp<S, SP>(src + (size_t)u[r] * SP, bsrc[r]);
+        }
     }
     /* slot phase: gather, send, write, delta, stage in LDS */
 #pragma unroll
@@ -685,11 +816,15 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                     }
                 }
                 if (MODE == ITER || MODE == INIT || MODE == EDGE) {
-                    float row[SP];
+                    if constexpr (PACKED) {
+                        store_crow<S>(a.msg_out + mrow[r], x, m);
+                    } else {
+                        float row[SP];
 #pragma unroll
-                    for (int i = 0; i < SP; ++i)
-                        row[i] = (i < S) ? m[i] : 0.0f;
-                    store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
+                        for (int i = 0; i < SP; ++i)
+                            row[i] = (i < S) ? m[i] : 0.0f;
+                        store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
+                    }
                 }
             }
 #pragma unroll
@@ -703,7 +838,8 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
     for (int r = 0; r < R; ++r) {
         unsigned v = nb + r * WAVE + lane;
         if (v < ne)
-            node_update<S, MIXED, MODE>(a, v, q0[r], q1[r], st[r], nn[r], lds, W);
+            node_update<S, MIXED, MODE, PACKED>(a, PACKED ? (size_t)nrow[r] : (size_t)v * SP, q0[r], q1[r], st[r],
+                                                nn[r], lds, W);
     }
 }
 
@@ -715,7 +851,8 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
  * one lane per slot would read a whole table alone. */
 template <int S, bool MIXED, int MODE, int L, bool PACKED>
 __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se,
-                                           float *lds, float *stage_f, unsigned *stage_u, unsigned lane, double &dacc)
+                                           float *lds, float *stage_f, unsigned *stage_u, unsigned lane, double &dacc,
+                                           uint2 cbase)
 {
     constexpr int SP = Stride<S>::value;
     constexpr int W = WAVE;
@@ -726,40 +863,68 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
     float st[S];
     /* stage every slot's operands once, one lane per slot (coalesced, and
      * the gathers issued together): the source's row b, the old message,
-     * the table reference and the state counts */
-    {
-        unsigned s = sb + lane;
-        if (s < se) {
-            unsigned xy = MIXED ? a.slot_xy[s] : (unsigned)(S | (S << 4));
-            unsigned t = (PACKED || MIXED) ? a.slot_tab[s] : 0u;
-            float b[S], old[S];
-            if (MODE == PREP) {
-                load_rowp<S, SP>(a.msg_in + (size_t)s * SP, b);
-            } else {
-                unsigned u = a.slot_src[s];
-                const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
-                load_rowp<S, SP>(src + (size_t)u * SP, b);
-            }
-            if (MODE == ITER || MODE == EDGE)
-                load_rowp<S, SP>(a.msg_in + (size_t)s * SP, old);
-#pragma unroll
-            for (int j = 0; j < S; ++j) {
-                stage_f[j * W + lane] = b[j];
-                if (MODE == ITER || MODE == EDGE)
-                    stage_f[(S + j) * W + lane] = old[j];
-            }
-            stage_u[lane] = xy;
-            stage_u[W + lane] = t;
-        }
+     * the table reference and the state counts.  The loads that depend on
+     * nothing come first; the compact rows' offsets (PACKED) are prefix sums
+     * of the state counts they bring, and the row loads follow. */
+    const unsigned s = sb + lane;
+    const bool slot_ok = s < se;
+    unsigned xy = (unsigned)(S | (S << 4)), t = 0u, u = 0u;
+    if (slot_ok) {
+        if (MIXED)
+            xy = a.slot_xy[s];
+        if (PACKED || MIXED)
+            t = a.slot_tab[s];
+        if (MODE != PREP)
+            u = a.slot_src[s];
     }
     if (v < ne) {
         q0 = a.in_off[v] - sb;
         q1 = a.in_off[v + 1] - sb;
-        if (MODE == ITER || MODE == PREP || MODE == FWD)
-            load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st);
-        else
-            load_rowp<S, SP>(a.states + (size_t)v * SP, st);
         n = MIXED ? a.nvars[v] : S;
+    }
+    unsigned mrow = 0, nrow = 0;
+    if constexpr (PACKED) {
+        mrow = cbase.y + wave_excl_scan(slot_ok ? even_width(xy & 15u) : 0u, lane);
+        nrow = cbase.x + wave_excl_scan((v < ne) ? even_width(n) : 0u, lane);
+    }
+    if (slot_ok) {
+        float b[S], old[S];
+        if (MODE == PREP) {
+            if constexpr (PACKED)
+                load_crow_z<S>(a.msg_in + mrow, xy & 15u, b);
+            else
+                load_rowp<S, SP>(a.msg_in + (size_t)s * SP, b);
+        } else {
+            const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
+            if constexpr (PACKED)
+                load_crow_z<S>(src + u, xy & 15u, b); /* the source's row: x entries */
+            else
+                load_rowp<S, SP>(src + (size_t)u * SP, b);
+        }
+        if (MODE == ITER || MODE == EDGE) {
+            if constexpr (PACKED)
+                load_crow<S>(a.msg_in + mrow, old);
+            else
+                load_rowp<S, SP>(a.msg_in + (size_t)s * SP, old);
+        }
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            stage_f[j * W + lane] = b[j];
+            if (MODE == ITER || MODE == EDGE)
+                stage_f[(S + j) * W + lane] = old[j];
+        }
+        stage_u[lane] = xy;
+        stage_u[W + lane] = t;
+    }
+    if (v < ne) {
+        if constexpr (PACKED) {
+            load_crow<S>(a.states + nrow, st);
+        } else {
+            if (MODE == ITER || MODE == PREP || MODE == FWD)
+                load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st);
+            else
+                load_rowp<S, SP>(a.states + (size_t)v * SP, st);
+        }
     }
     wave_sync();
     /* The table entries of pass p that this lane multiplies: rows
@@ -877,7 +1042,7 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                     }
                     lds[i * W + q] = m[r];
                 }
-                if ((MODE == ITER || MODE == INIT || MODE == EDGE) && !BP_ROWS_MSG_FROM_LDS)
+                if ((MODE == ITER || MODE == INIT || MODE == EDGE) && !BP_ROWS_MSG_FROM_LDS && !PACKED)
                     a.msg_out[(size_t)s * SP + i] = (i < (unsigned)S) ? m[r] : 0.0f;
             }
         }
@@ -889,7 +1054,7 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
         run_pass(p, tv);
     }
     wave_sync();
-    if ((MODE == ITER || MODE == INIT || MODE == EDGE) && BP_ROWS_MSG_FROM_LDS) {
+    if ((MODE == ITER || MODE == INIT || MODE == EDGE) && (BP_ROWS_MSG_FROM_LDS || PACKED)) {
         /* the new messages, one whole row per slot (zeros beyond the
          * message's entries are in LDS already) */
         unsigned s = sb + lane;
@@ -898,11 +1063,14 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
 #pragma unroll
             for (int i = 0; i < SP; ++i)
                 row[i] = (i < S) ? lds[i * W + lane] : 0.0f;
-            store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
+            if constexpr (PACKED)
+                store_crow<S>(a.msg_out + mrow, stage_u[lane] & 15u, row);
+            else
+                store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
         }
     }
     if (v < ne)
-        node_update<S, MIXED, MODE>(a, v, q0, q1, st, n, lds, W);
+        node_update<S, MIXED, MODE, PACKED>(a, PACKED ? (size_t)nrow : (size_t)v * SP, q0, q1, st, n, lds, W);
 }
 
 /* One node with more than 64 in-slots: its slots are walked in blocks of 64
@@ -912,17 +1080,29 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
  * other lanes wrote. */
 template <int S, bool MIXED, int MODE, bool PACKED>
 __device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *lds, unsigned lane, double &dacc,
-                          bool first, float *acc, unsigned n)
+                          bool first, float *acc, unsigned n, unsigned mbase)
 {
     float m[S];
     for (unsigned base = sb; base < se; base += WAVE) {
         unsigned s = base + lane;
+        unsigned mrow = 0;
+        if constexpr (PACKED) {
+            /* this rare path (a node with more in-slots than a chunk) sums
+             * the widths serially: a prefix scan here would raise the whole
+             * kernel's register count */
+            unsigned end = (se - base < (unsigned)WAVE) ? se : base + WAVE;
+            for (unsigned t = base; t < end; ++t) {
+                if (t == s)
+                    mrow = mbase;
+                mbase += even_width(a.slot_xy[t] & 15u);
+            }
+        }
         if (s < se) {
             if (first) {
-                slot_message<S, MIXED, MODE, PACKED>(a, s, m, dacc);
+                slot_message<S, MIXED, MODE, PACKED>(a, s, m, dacc, mrow);
             } else {
                 unsigned x;
-                compute_message<S, MIXED, MODE, PACKED>(a, s, m, x);
+                compute_message<S, MIXED, MODE, PACKED>(a, s, m, x, mrow);
             }
 #pragma unroll
             for (int k = 0; k < S; ++k)
@@ -944,17 +1124,23 @@ __device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *l
 
 template <int S, bool MIXED, int MODE, bool PACKED>
 __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned se, float *lds, unsigned lane,
-                           double &dacc)
+                           double &dacc, uint2 cbase)
 {
     constexpr int SP = Stride<S>::value;
     unsigned n = MIXED ? a.nvars[v] : S;
+    const size_t row = PACKED ? (size_t)cbase.x : (size_t)v * SP;
     float st[S], acc[S];
+    if constexpr (PACKED) {
+        load_crow<S>(a.states + row, st);
+    } else {
 #pragma unroll
-    for (int k = 0; k < S; ++k) {
-        st[k] = a.states[(size_t)v * SP + k];
-        acc[k] = (MODE == ITER || MODE == FWD) ? 1.0f : st[k];
+        for (int k = 0; k < S; ++k)
+            st[k] = a.states[row + k];
     }
-    walk_wide<S, MIXED, MODE, PACKED>(a, sb, se, lds, lane, dacc, true, acc, n);
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        acc[k] = (MODE == ITER || MODE == FWD) ? 1.0f : st[k];
+    walk_wide<S, MIXED, MODE, PACKED>(a, sb, se, lds, lane, dacc, true, acc, n, cbase.y);
     if (MODE == ITER || MODE == FWD) {
 #pragma unroll
         for (int k = 0; k < S; ++k)
@@ -964,12 +1150,12 @@ __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned
 #pragma unroll
         for (int k = 0; k < S; ++k)
             acc[k] = st[k];
-        walk_wide<S, MIXED, MODE, PACKED>(a, sb, se, lds, lane, dacc, false, acc, n);
+        walk_wide<S, MIXED, MODE, PACKED>(a, sb, se, lds, lane, dacc, false, acc, n, cbase.y);
     }
     if (MODE == EDGE)
         normalise_positive<S, MIXED>(acc, n);
     if (lane == 0)
-        write_node<S, MIXED, MODE>(a, v, st, acc, n);
+        write_node<S, MIXED, MODE, PACKED>(a, row, st, acc, n);
 }
 
 /* Slots per lane of the narrow path: two for small state counts. */
@@ -985,12 +1171,18 @@ __device__ __forceinline__ void run_chunk(const LoopyArgs &a, unsigned c, float 
 {
     uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
     unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
+    uint2 cbase = make_uint2(0u, 0u);
+    if constexpr (PACKED)
+        cbase = a.chunk_base[c];
     if (L > 1 && se - sb <= (unsigned)WAVE)
-        chunk_rows<S, MIXED, MODE, L, PACKED>(a, nb, ne, sb, se, lds, stage_f, stage_u, lane, dacc);
+        chunk_rows<S, MIXED, MODE, L, PACKED>(a, nb, ne, sb, se, lds, stage_f, stage_u, lane, dacc, cbase);
     else if (L == 1 && se - sb <= (unsigned)(WAVE * R))
-        chunk_narrow<S, MIXED, MODE, R, PACKED>(a, nb, ne, sb, se, lds, lane, dacc);
-    else
-        chunk_wide<S, MIXED, MODE, PACKED>(a, nb, sb, se, lds, lane, dacc);
+        chunk_narrow<S, MIXED, MODE, R, PACKED>(a, nb, ne, sb, se, lds, lane, dacc, cbase);
+    else if constexpr (!PACKED)
+        chunk_wide<S, MIXED, MODE, PACKED>(a, nb, sb, se, lds, lane, dacc, cbase);
+    /* (PACKED: a node with more in-slots than a chunk holds runs in
+     * k_loopy_wide after this launch, so the rare one-node path does not
+     * set this kernel's register budget) */
     wave_sync();
 }
 
@@ -1057,6 +1249,26 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
         if (lane == 0)
             a.partials[blockIdx.x * WAVES_PER_BLOCK + wave] = d;
     }
+}
+
+/* The chunks of a batch (PACKED) that hold one node with more in-slots than
+ * a chunk of the current shape takes: one wave per chunk, launched after
+ * k_loopy over the same chunks (which leaves their delta partials at 0). */
+template <int S, bool MIXED, int MODE>
+__global__ __launch_bounds__(BLOCK) void k_loopy_wide(LoopyArgs a, const unsigned *wide, unsigned nwide)
+{
+    __shared__ float lds[WAVES_PER_BLOCK][S * WAVE];
+    unsigned wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    unsigned i = blockIdx.x * WAVES_PER_BLOCK + wave;
+    if (i >= nwide)
+        return;
+    unsigned c = wide[i];
+    uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
+    double dacc = 0.0;
+    chunk_wide<S, MIXED, MODE, true>(a, lo.x, lo.y, hi.y, lds[wave], lane, dacc, a.chunk_base[c]);
+    wave_sync();
+    if (MODE == ITER || MODE == EDGE)
+        flush_chunk_partial(a, c, lane, dacc);
 }
 
 } // namespace bpk
