@@ -644,13 +644,28 @@ template <int S, bool MIXED, int MODE, bool PACKED = false>
 __device__ __forceinline__ void node_update(const LoopyArgs &a, size_t row, unsigned q0, unsigned q1, float *st,
                                             unsigned n, const float *lds, int W)
 {
-    float m[S], b[S];
+    /* the first K in-messages are read from LDS once, all together, and
+     * kept for both products (most nodes have no more); the rest are read
+     * in slot order as they are combined */
+    constexpr int K = (S <= 4) ? 4 : 2;
+    const unsigned cnt = q1 - q0;
+    float mk[K][S], m[S], b[S];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if ((unsigned)j < cnt)
+#pragma unroll
+            for (int k = 0; k < S; ++k)
+                mk[j][k] = lds[k * W + q0 + j];
     if (MODE == ITER || MODE == FWD) {
         float t[S];
 #pragma unroll
         for (int k = 0; k < S; ++k)
             t[k] = 1.0f;
-        for (unsigned q = q0; q < q1; ++q) {
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if ((unsigned)j < cnt)
+                combine<S, MIXED>(t, mk[j], n);
+        for (unsigned q = q0 + K; q < q1; ++q) {
 #pragma unroll
             for (int k = 0; k < S; ++k)
                 m[k] = lds[k * W + q];
@@ -667,7 +682,11 @@ __device__ __forceinline__ void node_update(const LoopyArgs &a, size_t row, unsi
 #pragma unroll
     for (int k = 0; k < S; ++k)
         b[k] = st[k];
-    for (unsigned q = q0; q < q1; ++q) {
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if ((unsigned)j < cnt)
+            combine_mode<S, MIXED, MODE>(b, mk[j], n);
+    for (unsigned q = q0 + K; q < q1; ++q) {
 #pragma unroll
         for (int k = 0; k < S; ++k)
             m[k] = lds[k * W + q];
@@ -1197,8 +1216,17 @@ __device__ __forceinline__ void flush_chunk_partial(const LoopyArgs &a, unsigned
 }
 
 /* L > 1: L lanes per slot over chunks of at most 64 slots and nodes (R = 1) */
+/* Minimum waves per SIMD requested of the compiler for k_loopy (0: its own
+ * choice); a compile-time switch for measuring register-bound shapes. */
+#ifndef BP_MIN_WAVES
+#define BP_MIN_WAVES 0
+#endif
 template <int S, bool MIXED, int MODE, int R = SlotsPerLane<S>::value, bool PACKED = false, int L = 1>
+#if BP_MIN_WAVES > 0
+__global__ __launch_bounds__(BLOCK, BP_MIN_WAVES) void k_loopy(LoopyArgs a)
+#else
 __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
+#endif
 {
     if (a.stop && *a.stop)
         return;
