@@ -81,6 +81,21 @@ namespace bpk {
 #ifndef BP_ROWS_MSG_FROM_LDS
 #define BP_ROWS_MSG_FROM_LDS 1
 #endif
+/* Lanes-per-slot path: 1 loads each node's belief after the passes instead
+ * of before them (8 fewer registers held through the passes at S = 8);
+ * always so for a kernel given a register budget below. */
+#ifndef BP_ROWS_LATE_STATES
+#define BP_ROWS_LATE_STATES 0
+#endif
+/* Minimum waves per SIMD requested of the compiler for k_loopy (1: its own
+ * choice).  Measuring switches: BP_MIN_WAVES for every k_loopy,
+ * BP_BATCH_MIN_WAVES for the batch kernels (PACKED) alone. */
+#ifndef BP_MIN_WAVES
+#define BP_MIN_WAVES 0
+#endif
+#ifndef BP_BATCH_MIN_WAVES
+#define BP_BATCH_MIN_WAVES 0
+#endif
 
 constexpr int WAVE = 64;
 constexpr int BLOCK = BP_BLOCK;
@@ -131,6 +146,18 @@ struct LoopyArgs {
 
 template <int S> struct Stride {
     static constexpr int value = (S <= 2) ? S : (S <= 4) ? 4 : (S <= 8) ? 8 : 10;
+};
+
+/* The batch kernel for up to 8 states with 2 lanes per slot needs 90
+ * registers, which leaves room for 5 waves per SIMD; held to 80 (6 waves,
+ * no spills once the belief is loaded after the passes) its iteration on
+ * mixed 2-8 state batches is 7% shorter.  Other shapes keep the compiler's
+ * choice: a budget of 6 waves makes several of them spill. */
+template <bool PACKED, int S, int L> struct MinWaves {
+    static constexpr int value = (PACKED && BP_BATCH_MIN_WAVES > 0) ? BP_BATCH_MIN_WAVES
+                                 : (BP_MIN_WAVES > 0)                ? BP_MIN_WAVES
+                                 : (PACKED && S == 8 && L == 2)      ? 6
+                                                                     : 1;
 };
 
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -935,23 +962,29 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
         stage_u[lane] = xy;
         stage_u[W + lane] = t;
     }
-    if (v < ne) {
-        if constexpr (PACKED) {
-            load_crow<S>(a.states + nrow, st);
-        } else {
-            if (MODE == ITER || MODE == PREP || MODE == FWD)
-                load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st);
-            else
-                load_rowp<S, SP>(a.states + (size_t)v * SP, st);
+    auto load_states = [&]() {
+        if (v < ne) {
+            if constexpr (PACKED) {
+                load_crow<S>(a.states + nrow, st);
+            } else {
+                if (MODE == ITER || MODE == PREP || MODE == FWD)
+                    load_rowp_nt<S, SP>(a.states + (size_t)v * SP, st);
+                else
+                    load_rowp<S, SP>(a.states + (size_t)v * SP, st);
+            }
         }
-    }
+    };
+    constexpr bool late_states = BP_ROWS_LATE_STATES || MinWaves<PACKED, S, L>::value > 1;
+    if (!late_states)
+        load_states();
     wave_sync();
     /* The table entries of pass p that this lane multiplies: rows
      * i = k, k + L, ... and columns j < y (zero elsewhere).  They are all
      * loaded before the first product, so the pass waits on memory once
      * rather than once per row (1.5x on mixed 2-8 state batches; loading
-     * the next pass's entries during this pass as well was no faster, the
-     * extra registers costing occupancy). */
+     * the next pass's entries during this pass as well, or the first pass's
+     * together with the staged rows, was no faster, the extra registers
+     * costing occupancy). */
     auto load_pass = [&](int p, float (&tv)[NR][S]) {
         unsigned q = p * G + g, s = sb + q;
 #pragma unroll
@@ -1088,6 +1121,8 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                 store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
         }
     }
+    if (late_states)
+        load_states();
     if (v < ne)
         node_update<S, MIXED, MODE, PACKED>(a, PACKED ? (size_t)nrow : (size_t)v * SP, q0, q1, st, n, lds, W);
 }
@@ -1215,26 +1250,22 @@ __device__ __forceinline__ void flush_chunk_partial(const LoopyArgs &a, unsigned
     dacc = 0.0;
 }
 
-/* L > 1: L lanes per slot over chunks of at most 64 slots and nodes (R = 1) */
-/* Minimum waves per SIMD requested of the compiler for k_loopy (0: its own
- * choice); a compile-time switch for measuring register-bound shapes. */
-#ifndef BP_MIN_WAVES
-#define BP_MIN_WAVES 0
-#endif
 template <int S, bool MIXED, int MODE, int R = SlotsPerLane<S>::value, bool PACKED = false, int L = 1>
-#if BP_MIN_WAVES > 0
-__global__ __launch_bounds__(BLOCK, BP_MIN_WAVES) void k_loopy(LoopyArgs a)
-#else
-__global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
-#endif
+__global__ __launch_bounds__(BLOCK, (MinWaves<PACKED, S, L>::value)) void k_loopy(LoopyArgs a)
 {
     if (a.stop && *a.stop)
         return;
-    __shared__ float lds[WAVES_PER_BLOCK][S * WAVE * R];
     /* lanes-per-slot path: staged slot operands (rows b and old, state counts, tables) */
     __shared__ float stage_f[WAVES_PER_BLOCK][L > 1 ? 2 * S * WAVE : 1];
     __shared__ unsigned stage_u[WAVES_PER_BLOCK][L > 1 ? 2 * WAVE : 1];
+    /* the new messages.  With L > 1 they take the place of the staged rows
+     * b (the first S * WAVE floats of stage_f): a slot's message is written
+     * by the instructions after those that read the slot's row b, and no
+     * other slot reads that row, so 8 KB less LDS per workgroup (S = 8)
+     * lets 8 workgroups share a CU instead of 6. */
+    __shared__ float lds_own[WAVES_PER_BLOCK][L > 1 ? 1 : S * WAVE * R];
     unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE), lane = threadIdx.x % WAVE;
+    float *lds_w = (L > 1) ? stage_f[wave] : lds_own[wave];
     double dacc = 0.0;
     const bool per_chunk = (MODE == ITER || MODE == EDGE) && a.chunk_partials != nullptr;
     if (a.queue) {
@@ -1250,7 +1281,7 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
             unsigned end = base + a.qblock < a.nchunks ? base + a.qblock : a.nchunks;
             for (unsigned p = base; p < end; ++p) {
                 unsigned c = a.order ? a.order[p] : p;
-                run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], stage_f[wave], stage_u[wave], lane, dacc);
+                run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds_w, stage_f[wave], stage_u[wave], lane, dacc);
                 if (per_chunk)
                     flush_chunk_partial(a, c, lane, dacc);
             }
@@ -1265,7 +1296,7 @@ __global__ __launch_bounds__(BLOCK) void k_loopy(LoopyArgs a)
         unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / W);
         for (unsigned p = c0; p < c1; ++p) {
             unsigned c = a.order ? a.order[p] : p;
-            run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds[wave], stage_f[wave], stage_u[wave], lane, dacc);
+            run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds_w, stage_f[wave], stage_u[wave], lane, dacc);
             if (per_chunk)
                 flush_chunk_partial(a, c, lane, dacc);
         }
