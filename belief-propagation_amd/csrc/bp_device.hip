@@ -143,6 +143,27 @@ extern "C" int bp_device_count(void)
     return n;
 }
 
+/* f(begin, end) over [0, n) split across up to 16 host threads (the
+ * graph's host arrays are MAX_STATES-strided: the repacking is bound by
+ * memory latency, not arithmetic) */
+template <typename F> static void par_for(size_t n, F f)
+{
+    unsigned T = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < (1u << 16) || T == 1) {
+        f((size_t)0, n);
+        return;
+    }
+    size_t step = (n + T - 1) / T;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < T; ++t) {
+        size_t a = t * step, b = std::min(n, a + step);
+        if (a < b)
+            th.emplace_back(f, a, b);
+    }
+    for (auto &x : th)
+        x.join();
+}
+
 /* Chunks: consecutive nodes with at most cap in-slots and cap nodes (cap =
  * 64 x slots per lane), or one node with more in-slots.  Each entry holds
  * the chunk's first node and first slot. */
@@ -291,30 +312,43 @@ void bp_cm_order(Graph_t g, std::vector<unsigned> &order)
  * same XCDs share lines.  Every node keeps its in-edges in order, so
  * results do not change.  The grouping is kept only when it lowers the
  * line floor (on a lattice, whose gathers are local, it does not). */
-static uint64_t line_floor(Graph_t g, const std::vector<unsigned> &inv, const std::vector<unsigned> &xcd_of_pos,
-                           unsigned SP)
+static uint64_t line_floor(Graph_t g, const std::vector<unsigned> &inv, const std::vector<unsigned> &perm,
+                           const unsigned bounds[9], unsigned SP)
 {
     unsigned V = g->current_num_vertices;
-    std::vector<unsigned> perm(V);
-    for (unsigned i = 0; i < V; ++i)
-        perm[inv[i]] = i;
     size_t nlines = ((size_t)V * SP * 4 + 127) / 128 + 1;
-    std::vector<unsigned char> seen(nlines * 8, 0);
+    uint64_t count[8] = {};
+    /* one thread per XCD: the distinct lines its nodes' sources occupy */
+    std::vector<std::thread> th;
+    for (unsigned x = 0; x < 8; ++x)
+        th.emplace_back([&, x]() {
+            std::vector<unsigned char> seen(nlines, 0);
+            uint64_t n = 0;
+            for (unsigned i = bounds[x]; i < bounds[x + 1]; ++i) {
+                unsigned h = inv[i];
+                for (unsigned k = g->dest_nodes_to_edges_node_list[h];
+                     k < bp_csr_end(g, g->dest_nodes_to_edges_node_list, h); ++k) {
+                    unsigned e = g->dest_nodes_to_edges_edge_list[k];
+                    size_t l = (size_t)perm[g->edges_src_index[e]] * SP * 4 / 128;
+                    if (!seen[l]) {
+                        seen[l] = 1;
+                        ++n;
+                    }
+                }
+            }
+            count[x] = n;
+        });
+    for (auto &t : th)
+        t.join();
     uint64_t lines = 0;
-    for (unsigned e = 0; e < g->current_num_edges; ++e) {
-        unsigned x = xcd_of_pos[perm[g->edges_dest_index[e]]];
-        size_t l = (size_t)perm[g->edges_src_index[e]] * SP * 4 / 128;
-        if (!seen[l * 8 + x]) {
-            seen[l * 8 + x] = 1;
-            ++lines;
-        }
-    }
+    for (unsigned x = 0; x < 8; ++x)
+        lines += count[x];
     return lines;
 }
 
-/* XCD of each device position under the iteration kernel's chunk split */
-static void xcd_ranges(Graph_t g, const std::vector<unsigned> &inv, unsigned cap, std::vector<unsigned> &xcd_of_pos,
-                       unsigned bounds[9])
+/* First device position of each XCD's range under the iteration kernel's
+ * chunk split (bounds[8] = V) */
+static void xcd_ranges(Graph_t g, const std::vector<unsigned> &inv, unsigned cap, unsigned bounds[9])
 {
     unsigned V = g->current_num_vertices;
     std::vector<unsigned> in_off(V + 1, 0);
@@ -328,12 +362,17 @@ static void xcd_ranges(Graph_t g, const std::vector<unsigned> &inv, unsigned cap
     unsigned nch = (unsigned)chunks.size() - 1;
     unsigned want = (nch + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
     unsigned long long W = (unsigned long long)std::max(8u, (want + 7u) & ~7u) * WAVES_PER_BLOCK;
-    xcd_of_pos.assign(V, 0);
     for (unsigned x = 0; x <= 8; ++x)
         bounds[x] = chunks[(unsigned)((x * W / 8) * nch / W)].x;
-    for (unsigned x = 0; x < 8; ++x)
-        for (unsigned i = bounds[x]; i < bounds[x + 1]; ++i)
-            xcd_of_pos[i] = x;
+}
+
+static void inverse_of(const std::vector<unsigned> &inv, std::vector<unsigned> &perm)
+{
+    perm.resize(inv.size());
+    par_for(inv.size(), [&](size_t a, size_t b) {
+        for (size_t i = a; i < b; ++i)
+            perm[inv[i]] = (unsigned)i;
+    });
 }
 
 static void xcd_group(Graph_t g, std::vector<unsigned> &inv, unsigned cap, unsigned SP)
@@ -341,25 +380,61 @@ static void xcd_group(Graph_t g, std::vector<unsigned> &inv, unsigned cap, unsig
     unsigned V = g->current_num_vertices;
     if (V < 2)
         return;
-    std::vector<unsigned> xpos;
     unsigned bounds[9];
-    xcd_ranges(g, inv, cap, xpos, bounds);
-    std::vector<unsigned> perm(V);
-    for (unsigned i = 0; i < V; ++i)
-        perm[inv[i]] = i;
-    std::vector<unsigned char> mask(V, 0);
-    for (unsigned e = 0; e < g->current_num_edges; ++e)
-        mask[g->edges_src_index[e]] |= (unsigned char)(1u << xpos[perm[g->edges_dest_index[e]]]);
-    std::vector<unsigned> grouped(inv);
-    for (unsigned x = 0; x < 8; ++x) {
-        auto key = [&](unsigned v) { return mask[v] == (1u << x) ? -1 : (int)mask[v]; };
-        std::stable_sort(grouped.begin() + bounds[x], grouped.begin() + bounds[x + 1],
-                         [&](unsigned a, unsigned b) { return key(a) < key(b); });
+    xcd_ranges(g, inv, cap, bounds);
+    std::vector<unsigned> perm;
+    inverse_of(inv, perm);
+    /* the XCDs that gather each node (those of its out-edges' destinations):
+     * one thread per XCD marks the sources of its nodes' in-edges */
+    std::vector<unsigned char> marks((size_t)8 * V, 0), mask(V, 0);
+    {
+        std::vector<std::thread> th;
+        for (unsigned x = 0; x < 8; ++x)
+            th.emplace_back([&, x]() {
+                unsigned char *mk = marks.data() + (size_t)x * V;
+                for (unsigned i = bounds[x]; i < bounds[x + 1]; ++i) {
+                    unsigned h = inv[i];
+                    for (unsigned k = g->dest_nodes_to_edges_node_list[h];
+                         k < bp_csr_end(g, g->dest_nodes_to_edges_node_list, h); ++k)
+                        mk[g->edges_src_index[g->dest_nodes_to_edges_edge_list[k]]] = 1;
+                }
+            });
+        for (auto &t : th)
+            t.join();
     }
-    std::vector<unsigned> xpos2;
+    par_for(V, [&](size_t a, size_t b) {
+        for (size_t h = a; h < b; ++h) {
+            unsigned m = 0;
+            for (unsigned x = 0; x < 8; ++x)
+                m |= (unsigned)marks[(size_t)x * V + h] << x;
+            mask[h] = (unsigned char)m;
+        }
+    });
+    /* within each XCD's range a stable counting sort by mask, the nodes
+     * only that XCD gathers first */
+    std::vector<unsigned> grouped(V);
+    {
+        std::vector<std::thread> th;
+        for (unsigned x = 0; x < 8; ++x)
+            th.emplace_back([&, x]() {
+                auto key = [&](unsigned v) { return mask[v] == (1u << x) ? 0u : 1u + mask[v]; };
+                unsigned start[258] = {};
+                for (unsigned i = bounds[x]; i < bounds[x + 1]; ++i)
+                    ++start[key(inv[i]) + 1];
+                start[0] = bounds[x];
+                for (unsigned k = 1; k < 258; ++k)
+                    start[k] += start[k - 1];
+                for (unsigned i = bounds[x]; i < bounds[x + 1]; ++i)
+                    grouped[start[key(inv[i])]++] = inv[i];
+            });
+        for (auto &t : th)
+            t.join();
+    }
     unsigned bounds2[9];
-    xcd_ranges(g, grouped, cap, xpos2, bounds2);
-    if (line_floor(g, grouped, xpos2, SP) * 100 < line_floor(g, inv, xpos, SP) * 99)
+    xcd_ranges(g, grouped, cap, bounds2);
+    std::vector<unsigned> perm2;
+    inverse_of(grouped, perm2);
+    if (line_floor(g, grouped, perm2, bounds2, SP) * 100 < line_floor(g, inv, perm, bounds, SP) * 99)
         inv.swap(grouped);
 }
 
@@ -406,27 +481,6 @@ static int check_graph(Graph_t g)
             return -1;
         }
     return 0;
-}
-
-/* f(begin, end) over [0, n) split across up to 16 host threads (the
- * graph's host arrays are MAX_STATES-strided: the repacking is bound by
- * memory latency, not arithmetic) */
-template <typename F> static void par_for(size_t n, F f)
-{
-    unsigned T = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
-    if (n < (1u << 16) || T == 1) {
-        f((size_t)0, n);
-        return;
-    }
-    size_t step = (n + T - 1) / T;
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < T; ++t) {
-        size_t a = t * step, b = std::min(n, a + step);
-        if (a < b)
-            th.emplace_back(f, a, b);
-    }
-    for (auto &x : th)
-        x.join();
 }
 
 /* the handle's pinned staging buffer, at least n floats */
@@ -478,6 +532,46 @@ static void unpack_rows(const float *in, const unsigned *valid, size_t count, un
     });
 }
 
+/* The shared table pool.  tables holds the per-slot tables [s][S*S]; the
+ * pool holds each table once, in the order of the first slot that reads
+ * it, and tab[s] is slot s's pool index, with bit 31 set where s is the
+ * later slot of a pair (partner[s] < s) whose table is exactly the
+ * transpose of its partner's, which it then reads transposed.  The earlier
+ * slot of a pair never reuses, so every slot's choice is independent. */
+static void build_table_pool(const std::vector<unsigned> &partner, const float *tables, unsigned E, unsigned S,
+                             std::vector<unsigned> &tab, std::vector<float> &pool)
+{
+    const size_t T = (size_t)S * S;
+    std::vector<unsigned char> reuse(E, 0);
+    par_for(E, [&](size_t a, size_t b) {
+        for (size_t s = a; s < b; ++s) {
+            unsigned f = partner[s];
+            if (f == ~0u || f >= s)
+                continue;
+            const float *A = tables + s * T, *Bt = tables + (size_t)f * T;
+            bool t = true;
+            for (unsigned i = 0; i < S && t; ++i)
+                for (unsigned j = 0; j < S && t; ++j)
+                    t = memcmp(&A[i * S + j], &Bt[j * S + i], 4) == 0;
+            reuse[s] = t;
+        }
+    });
+    tab.resize(E);
+    unsigned n = 0;
+    for (unsigned s = 0; s < E; ++s)
+        if (!reuse[s])
+            tab[s] = n++;
+    for (unsigned s = 0; s < E; ++s)
+        if (reuse[s])
+            tab[s] = tab[partner[s]] | 0x80000000u;
+    pool.resize((size_t)n * T);
+    par_for(E, [&](size_t a, size_t b) {
+        for (size_t s = a; s < b; ++s)
+            if (!reuse[s])
+                memcpy(pool.data() + (size_t)tab[s] * T, tables + s * T, T * sizeof(float));
+    });
+}
+
 static int upload_state(bp_dev *d, Graph_t g, bool tables)
 {
     size_t VS = (size_t)d->V * d->SP, ES = (size_t)d->E * d->SP;
@@ -525,31 +619,9 @@ static int upload_state(bp_dev *d, Graph_t g, bool tables)
         HIP_TRY(hipMemcpyAsync(d->J, tmp, d->jfloats * sizeof(float), hipMemcpyHostToDevice, d->stream));
         HIP_TRY(hipStreamSynchronize(d->stream));
         if (d->shared) {
-            /* tmp holds the per-slot tables [s][S*S]: each table once, in
-             * the order of the first slot that reads it; the later slot of
-             * a pair whose tables are exact transposes reads its partner's */
-            std::vector<unsigned> pool_of(d->E), tab(d->E);
+            std::vector<unsigned> tab;
             std::vector<float> pool;
-            pool.reserve((size_t)d->E * S * S);
-            unsigned n = 0;
-            for (unsigned s = 0; s < d->E; ++s) {
-                unsigned f = d->partner[s];
-                bool reuse = false;
-                if (f != ~0u && f < s) {
-                    const float *A = tmp + (size_t)s * S * S, *Bt = tmp + (size_t)f * S * S;
-                    reuse = true;
-                    for (unsigned i = 0; i < S && reuse; ++i)
-                        for (unsigned j = 0; j < S && reuse; ++j)
-                            reuse = memcmp(&A[i * S + j], &Bt[j * S + i], 4) == 0;
-                }
-                if (reuse && !(tab[f] >> 31)) {
-                    tab[s] = pool_of[f] | 0x80000000u;
-                } else {
-                    pool_of[s] = n;
-                    tab[s] = n++;
-                    pool.insert(pool.end(), tmp + (size_t)s * S * S, tmp + (size_t)(s + 1) * S * S);
-                }
-            }
+            build_table_pool(d->partner, tmp, d->E, S, tab, pool);
             HIP_TRY(hipMemcpy(d->Jp, pool.data(), pool.size() * sizeof(float), hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(d->slot_pool, tab.data(), (size_t)d->E * 4, hipMemcpyHostToDevice));
         }
@@ -567,24 +639,28 @@ static double algorithmic_bytes(Graph_t g)
      * per-node traffic of one iteration */
     double bytes = 0.0;
     unsigned E = g->current_num_edges, V = g->current_num_vertices;
-    std::vector<unsigned char> paired(E, 0);
-    /* a reverse edge emitted by the parsers directly follows its forward edge */
-    for (unsigned e = 0; e + 1 < E; ++e) {
-        if (paired[e])
-            continue;
-        unsigned f = e + 1;
-        if (g->edges_src_index[f] == g->edges_dest_index[e] && g->edges_dest_index[f] == g->edges_src_index[e]) {
+    /* a reverse edge emitted by the parsers directly follows its forward
+     * edge: rev[e] says edge e + 1 is e's reverse with the transposed table
+     * (tested on all host threads: the tables are 400 bytes apart) */
+    std::vector<unsigned char> rev(E, 0), paired(E, 0);
+    par_for(E ? E - 1 : 0, [&](size_t a, size_t b) {
+        for (size_t e = a; e < b; ++e) {
+            size_t f = e + 1;
+            if (g->edges_src_index[f] != g->edges_dest_index[e] || g->edges_dest_index[f] != g->edges_src_index[e])
+                continue;
             unsigned x = g->edges_x_dim[e], y = g->edges_y_dim[e];
-            const float *A = g->edges_joint_probabilities + (size_t)e * 100;
-            const float *B = g->edges_joint_probabilities + (size_t)f * 100;
+            const float *A = g->edges_joint_probabilities + e * 100;
+            const float *B = g->edges_joint_probabilities + f * 100;
             bool t = true;
             for (unsigned i = 0; i < x && t; ++i)
                 for (unsigned j = 0; j < y && t; ++j)
                     t = (memcmp(&A[i * 10 + j], &B[j * 10 + i], 4) == 0);
-            if (t)
-                paired[f] = 1;
+            rev[e] = t;
         }
-    }
+    });
+    for (unsigned e = 0; e + 1 < E; ++e)
+        if (!paired[e] && rev[e])
+            paired[e + 1] = 1;
     for (unsigned e = 0; e < E; ++e) {
         double x = g->edges_x_dim[e], y = g->edges_y_dim[e];
         if (!paired[e])
@@ -609,10 +685,24 @@ static void plan_table_sharing(bp_dev *d, Graph_t g, const std::vector<uint2> &c
 {
     unsigned E = d->E;
     d->partner.assign(E, ~0u);
-    std::vector<std::pair<unsigned long long, unsigned>> key(E);
-    for (unsigned e = 0; e < E; ++e)
-        key[e] = {((unsigned long long)g->edges_src_index[e] << 32) | g->edges_dest_index[e], e};
-    std::sort(key.begin(), key.end());
+    /* rev[e]: the lowest-numbered edge dest(e) -> src(e), found among the
+     * in-edges of src(e) (all host threads) */
+    std::vector<unsigned> rev(E, ~0u);
+    par_for(E, [&](size_t a, size_t b) {
+        for (size_t e = a; e < b; ++e) {
+            unsigned from = g->edges_src_index[e], to = g->edges_dest_index[e];
+            if (from == to)
+                continue;
+            unsigned best = ~0u;
+            for (unsigned k = g->dest_nodes_to_edges_node_list[from];
+                 k < bp_csr_end(g, g->dest_nodes_to_edges_node_list, from); ++k) {
+                unsigned f = g->dest_nodes_to_edges_edge_list[k];
+                if (g->edges_src_index[f] == to && f < best)
+                    best = f;
+            }
+            rev[e] = best;
+        }
+    });
     unsigned paired = 0, local = 0;
     unsigned nch = (unsigned)chunks.size() - 1;
     unsigned long long W = (unsigned long long)d->G * WAVES_PER_BLOCK;
@@ -626,11 +716,9 @@ static void plan_table_sharing(bp_dev *d, Graph_t g, const std::vector<uint2> &c
         return x;
     };
     for (unsigned e = 0; e < E; ++e) {
-        unsigned long long rev = ((unsigned long long)g->edges_dest_index[e] << 32) | g->edges_src_index[e];
-        auto it = std::lower_bound(key.begin(), key.end(), std::make_pair(rev, 0u));
-        if (it == key.end() || it->first != rev || g->edges_src_index[e] == g->edges_dest_index[e])
+        if (rev[e] == ~0u)
             continue;
-        unsigned s = d->edge_slot[e], f = d->edge_slot[it->second];
+        unsigned s = d->edge_slot[e], f = d->edge_slot[rev[e]];
         if (d->partner[s] != ~0u || d->partner[f] != ~0u)
             continue;
         d->partner[s] = f;
