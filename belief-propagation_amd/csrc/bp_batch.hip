@@ -507,9 +507,10 @@ extern "C" int bp_batch_upload(bp_batch *b, int device)
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     b->uploaded = true; /* from here on bp_batch_free releases device memory */
 
-    /* compact rows; S floats of slack past the last row, which the kernels'
-     * unpredicated row loads may read */
-    size_t VS = b->node_row_off.back() + b->S + 1, ES = b->msg_row_off.back() + b->S + 1;
+    /* compact rows; floats of slack past the last row, which the kernels'
+     * unpredicated row loads (up to 8 floats, or S) may read */
+    const size_t slack = std::max(b->S, 8u) + 1;
+    size_t VS = b->node_row_off.back() + slack, ES = b->msg_row_off.back() + slack;
     std::vector<float> st(VS, 0.0f);
     {
         size_t r = 0;
@@ -657,6 +658,12 @@ static void launch_batch_srl(int mode, const LoopyArgs &a, unsigned G, hipStream
     }
 }
 
+/* 1: batches of 5 to 7 states run the two-lane shape on the 8-state kernel
+ * (a compile-time switch so that the comparison can be repeated) */
+#ifndef BP_BATCH_ROUND_STATES
+#define BP_BATCH_ROUND_STATES 1
+#endif
+
 template <int S>
 static void launch_batch_s(unsigned R, unsigned L, int mode, const LoopyArgs &a, unsigned G, hipStream_t st,
                            const hipEvent_t *ev, const Shape &sh)
@@ -668,8 +675,15 @@ static void launch_batch_s(unsigned R, unsigned L, int mode, const LoopyArgs &a,
         }
     }
     if constexpr (S >= 4) {
+        /* two lanes per slot: 5 to 7 states run the 8-state kernel (every
+         * read is masked by the slot's and node's own counts, so the results
+         * are the same).  Compiled for 5-7 states that shape needs 115-159
+         * registers, for 8 states 80; on mixed 2-7 state batches its
+         * iteration is 1.33 ms against 2.38, and the best shape 8% faster.
+         * With 4 or 8 lanes the own-count kernels are the faster ones. */
+        constexpr int KS = (BP_BATCH_ROUND_STATES && S >= 5 && S <= 7) ? 8 : S;
         if (L == 2) {
-            launch_batch_srl<S, 1, 2>(mode, a, G, st, ev, sh);
+            launch_batch_srl<KS, 1, 2>(mode, a, G, st, ev, sh);
             return;
         }
         if (L == 4) {

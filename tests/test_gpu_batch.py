@@ -63,8 +63,12 @@ def test_batch_matches_plain_per_instance(lib):
     b.free()
 
 
-@pytest.mark.parametrize("preset_hi", [4, 8])
+@pytest.mark.parametrize("preset_hi", [4, 5, 7, 8])
 def test_every_shape_gives_the_same_bits(lib, preset_hi):
+    """Every launch shape gives the bits of the first, which are those of
+    the plain run; with 5 to 7 states the shape with two lanes per slot
+    runs the 8-state kernel."""
+    print(f"states 2-{preset_hi}", flush=True)
     b = B.Batch()
     gs = []
     for s in range(6):
@@ -89,6 +93,14 @@ def test_every_shape_gives_the_same_bits(lib, preset_hi):
             assert same_bits(got[0], ref[0]), f"shape {R},{L},{q}"
             assert np.array_equal(got[1], ref[1]), f"shape {R},{L},{q} deltas"
     b.free()
+    # the plain handle on the same network from its original beliefs
+    bn = B.generate("c4net", num_nodes=400 + 300 * 5, states_hi=preset_hi, seed=900 + 5)
+    g = B.bn_to_graph(bn)
+    lib.bp_bn_free(bn)
+    pd = plain_run(g, 12)
+    assert same_bits(B.states(g), ref[0]), "plain run"
+    assert np.allclose(ref[1][:, -1], pd, rtol=1e-12, atol=1e-300), "plain deltas"
+    lib.graph_destroy(g)
     for g in gs:
         lib.graph_destroy(g)
 
