@@ -961,39 +961,42 @@ template <int S> struct PlainLanes {
     static constexpr int value = (S >= 4) ? BP_PLAIN_L : 1;
 };
 
-template <int S, bool MIXED, int MODE>
+template <int S, bool MIXED, int MODE, bool POOLED>
 static void launch_k(const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
 {
     dim3 grid(G), block(BLOCK);
     if constexpr (PlainLanes<S>::value > 1) {
         constexpr int L = PlainLanes<S>::value;
         if (ev)
-            hipExtLaunchKernelGGL((k_loopy<S, MIXED, MODE, 1, false, L>), grid, block, 0, st, ev[0], ev[1], 0, a);
+            hipExtLaunchKernelGGL((k_loopy<S, MIXED, MODE, 1, false, L, POOLED>), grid, block, 0, st, ev[0], ev[1], 0,
+                                  a);
         else
-            hipLaunchKernelGGL((k_loopy<S, MIXED, MODE, 1, false, L>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_loopy<S, MIXED, MODE, 1, false, L, POOLED>), grid, block, 0, st, a);
     } else {
+        constexpr int R = SlotsPerLane<S>::value;
         if (ev)
-            hipExtLaunchKernelGGL((k_loopy<S, MIXED, MODE>), grid, block, 0, st, ev[0], ev[1], 0, a);
+            hipExtLaunchKernelGGL((k_loopy<S, MIXED, MODE, R, false, 1, POOLED>), grid, block, 0, st, ev[0], ev[1], 0,
+                                  a);
         else
-            hipLaunchKernelGGL((k_loopy<S, MIXED, MODE>), grid, block, 0, st, a);
+            hipLaunchKernelGGL((k_loopy<S, MIXED, MODE, R, false, 1, POOLED>), grid, block, 0, st, a);
     }
 }
 
-template <int S, bool MIXED>
+template <int S, bool MIXED, bool POOLED = false>
 static void launch_s(int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
 {
     if (mode == ITER)
-        launch_k<S, MIXED, ITER>(a, G, st, ev);
+        launch_k<S, MIXED, ITER, POOLED>(a, G, st, ev);
     else if (mode == INIT)
-        launch_k<S, MIXED, INIT>(a, G, st, ev);
+        launch_k<S, MIXED, INIT, POOLED>(a, G, st, ev);
     else if (mode == EDGE)
-        launch_k<S, MIXED, EDGE>(a, G, st, ev);
+        launch_k<S, MIXED, EDGE, POOLED>(a, G, st, ev);
     else if (mode == FWD)
-        launch_k<S, MIXED, FWD>(a, G, st, ev);
+        launch_k<S, MIXED, FWD, POOLED>(a, G, st, ev);
     else if (mode == FINIT)
-        launch_k<S, MIXED, FINIT>(a, G, st, ev);
+        launch_k<S, MIXED, FINIT, POOLED>(a, G, st, ev);
     else
-        launch_k<S, MIXED, PREP>(a, G, st, ev);
+        launch_k<S, MIXED, PREP, POOLED>(a, G, st, ev);
 }
 
 template <bool MIXED>
@@ -1013,12 +1016,25 @@ static void launch_m(unsigned S, int mode, const LoopyArgs &a, unsigned G, hipSt
     }
 }
 
+/* uniform state counts up to 4 whose transposed pairs share stored tables */
+static void launch_pooled(unsigned S, int mode, const LoopyArgs &a, unsigned G, hipStream_t st, const hipEvent_t *ev)
+{
+    switch (S) {
+    case 1: launch_s<1, false, true>(mode, a, G, st, ev); break;
+    case 2: launch_s<2, false, true>(mode, a, G, st, ev); break;
+    case 3: launch_s<3, false, true>(mode, a, G, st, ev); break;
+    default: launch_s<4, false, true>(mode, a, G, st, ev); break;
+    }
+}
+
 static void launch_on(bp_dev *d, int mode, const LoopyArgs &a, hipStream_t st, const hipEvent_t *ev = nullptr)
 {
     if (d->V == 0)
         return;
     if (d->mixed)
         launch_m<true>(d->S, mode, a, d->G, st, ev);
+    else if (d->shared && a.slot_tab)
+        launch_pooled(d->S, mode, a, d->G, st, ev);
     else
         launch_m<false>(d->S, mode, a, d->G, st, ev);
 }

@@ -293,22 +293,53 @@ template <int S>
 __device__ __forceinline__ void load_shared_table(const LoopyArgs &a, unsigned s, float *J)
 {
     unsigned t = a.slot_tab[s];
-    float T[S * S];
-    load_row<S * S>(a.J + (size_t)(t & 0x7fffffffu) * S * S, T);
+    const float *T = a.J + (size_t)(t & 0x7fffffffu) * S * S;
     const bool tr = (t >> 31) != 0;
+    /* each entry from its own address (a select between two entries of a
+     * loaded register array compiles to a dynamically indexed read) */
 #pragma unroll
     for (int i = 0; i < S; ++i)
 #pragma unroll
         for (int j = 0; j < S; ++j)
-            J[i * S + j] = tr ? T[j * S + i] : T[i * S + j];
+            J[i * S + j] = T[tr ? j * S + i : i * S + j];
+}
+
+/* A pooled uniform table as stored: S = 2 and 4 in aligned four-float
+ * vectors (the pool's tables are S*S floats, 16-byte aligned for those). */
+template <int S> struct PoolRaw {
+    static constexpr int NV = (S == 2 || S == 4) ? S * S / 4 : 1;
+    v4f v[NV];
+};
+
+template <int S> __device__ __forceinline__ void load_pool_raw(const float *T, PoolRaw<S> &p)
+{
+    if constexpr (S == 2 || S == 4) {
+#pragma unroll
+        for (int q = 0; q < PoolRaw<S>::NV; ++q)
+            p.v[q] = reinterpret_cast<const v4f *>(T)[q];
+    }
+}
+
+/* J = the stored table, or its transpose when tr: selects between
+ * register values, after the loads have arrived */
+template <int S> __device__ __forceinline__ void pool_table(const PoolRaw<S> &p, bool tr, float *J)
+{
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+        for (int j = 0; j < S; ++j) {
+            const int f = i * S + j, g = j * S + i;
+            float as = p.v[f / 4][f % 4], at = p.v[g / 4][g % 4];
+            J[f] = tr ? at : as;
+        }
 }
 
 /* The table of slot s as J[i*S + j] (zero beyond x rows and y columns). */
-template <int S, bool MIXED, bool PACKED>
+template <int S, bool MIXED, bool PACKED, bool POOLED>
 __device__ __forceinline__ void load_table(const LoopyArgs &a, unsigned s, unsigned xy, float *J)
 {
     if constexpr (!PACKED && !MIXED) {
-        if (BP_SHARE_TABLES && S <= 4 && a.slot_tab)
+        if constexpr (POOLED)
             load_shared_table<S>(a, s, J);
         else
             load_row_nt<S * S>(a.J + (size_t)s * S * S, J);
@@ -337,17 +368,15 @@ __device__ __forceinline__ void load_table(const LoopyArgs &a, unsigned s, unsig
 /* m = N(J . b) with the table read row by row from memory instead of held
  * in registers (large state counts); same operations in the same order as
  * send(). */
-template <int S, bool MIXED, bool PACKED>
+template <int S, bool MIXED, bool PACKED, bool POOLED>
 __device__ __forceinline__ void send_from_memory(const LoopyArgs &a, unsigned s, unsigned xy, const float *b, float *m)
 {
     unsigned x = MIXED ? (xy & 15u) : (unsigned)S, y = MIXED ? (xy >> 4) : (unsigned)S;
-    if constexpr (!PACKED && !MIXED && S <= 4) {
-        if (BP_SHARE_TABLES && a.slot_tab) {
-            float Js[S * S];
-            load_shared_table<S>(a, s, Js);
-            send<S, false>(Js, b, S, S, m);
-            return;
-        }
+    if constexpr (POOLED) {
+        float Js[S * S];
+        load_shared_table<S>(a, s, Js);
+        send<S, false>(Js, b, S, S, m);
+        return;
     }
     const float *T = PACKED ? a.tabs + a.slot_tab[s] : MIXED ? a.J + a.slot_tab[s] : a.J + (size_t)s * S * S;
     float sum = 0.0f;
@@ -450,7 +479,7 @@ __device__ __forceinline__ void store_crow(float *__restrict__ p, unsigned n, co
 
 /* The message of slot s into m[0..S) (zero beyond x), without side effects.
  * mrow: the slot's message row offset (PACKED). */
-template <int S, bool MIXED, int MODE, bool PACKED>
+template <int S, bool MIXED, int MODE, bool PACKED, bool POOLED>
 __device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, float *m, unsigned &x,
                                                 unsigned mrow = 0)
 {
@@ -484,18 +513,18 @@ __device__ __forceinline__ void compute_message(const LoopyArgs &a, unsigned s, 
         for (int j = 0; j < S; ++j)
             b[j] = src[(size_t)u * SP + j];
     }
-    send_from_memory<S, MIXED, PACKED>(a, s, x | (y << 4), b, m);
+    send_from_memory<S, MIXED, PACKED, POOLED>(a, s, x | (y << 4), b, m);
 }
 
 /* Slot phase: the message of slot s, written to msg_out (ITER, INIT, EDGE)
  * with its |new - old| added to dacc (ITER, EDGE). */
-template <int S, bool MIXED, int MODE, bool PACKED>
+template <int S, bool MIXED, int MODE, bool PACKED, bool POOLED>
 __device__ __forceinline__ void slot_message(const LoopyArgs &a, unsigned s, float *m, double &dacc,
                                              unsigned mrow = 0)
 {
     constexpr int SP = Stride<S>::value;
     unsigned x;
-    compute_message<S, MIXED, MODE, PACKED>(a, s, m, x, mrow);
+    compute_message<S, MIXED, MODE, PACKED, POOLED>(a, s, m, x, mrow);
     if (MODE == PREP || MODE == FWD || MODE == FINIT)
         return;
     if (MODE == ITER || MODE == EDGE) {
@@ -753,7 +782,7 @@ __device__ __forceinline__ void load_rowp_nt(const float *__restrict__ p, float 
  * slots sb+l+64r and nodes nb+l+64r (r < R).  Every load that does not
  * depend on another one is issued before any is consumed, so a wave keeps
  * R slots' and R nodes' worth of reads in flight across one round trip. */
-template <int S, bool MIXED, int MODE, int R, bool PACKED>
+template <int S, bool MIXED, int MODE, int R, bool PACKED, bool POOLED>
 __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se,
                                              float *lds, unsigned lane, double &dacc, uint2 base)
 {
@@ -762,8 +791,13 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
     /* small tables are loaded up front with the other streams; large ones
      * are read row by row when the message is formed (register budget) */
     constexpr bool HOLD = S * S * R <= 32;
-    unsigned u[R], xy[R], q0[R], q1[R], nn[R];
+    /* uniform state counts with a shared table pool (S = 2, 4): the slot's
+     * pool entry comes with the first loads and its table with the gathers,
+     * transposed in registers when the message is formed */
+    constexpr bool POOL = POOLED && HOLD && (S == 2 || S == 4);
+    unsigned u[R], xy[R], q0[R], q1[R], nn[R], tix[POOL ? R : 1];
     float J[HOLD ? R : 1][HOLD ? S * S : 1], old[R][S], st[R][S];
+    PoolRaw<S> praw[POOL ? R : 1];
     /* Lanes past the chunk's end read a valid neighbour instead (and drop
      * the result), so the loads carry no branches and all of them -- then
      * all the gathers -- are in flight before the first wait. */
@@ -771,25 +805,11 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
      * loads are skipped, uniformly for the wave) */
     const bool any_slots = se > sb;
     const unsigned s_last = se - 1, v_last = ne - 1;
+    /* the nodes' loads first, then the slots' in one block: the gathers'
+     * addresses are formed from the slots' loads, and the wait for them then
+     * comes after every first-round load has been issued */
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        unsigned s = sb + r * WAVE + lane;
-        if (s >= se)
-            s = s_last;
-        xy[r] = (unsigned)(S | (S << 4));
-        if (any_slots) {
-            if (MODE != PREP) {
-                u[r] = BP_NT_LOADS ? __builtin_nontemporal_load(a.slot_src + s) : a.slot_src[s];
-                if constexpr (MIXED)
-                    xy[r] = a.slot_xy[s];
-                if constexpr (HOLD)
-                    load_table<S, MIXED, PACKED>(a, s, xy[r], J[r]);
-            } else if constexpr (PACKED) {
-                xy[r] = a.slot_xy[s];
-            }
-            if (!PACKED && (MODE == ITER || MODE == EDGE || MODE == PREP))
-                load_rowp_nt<S, SP>(a.msg_in + (size_t)s * SP, old[r]);
-        }
         unsigned v = nb + r * WAVE + lane;
         if (v >= ne)
             v = v_last;
@@ -803,6 +823,28 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 load_rowp<S, SP>(a.states + (size_t)v * SP, st[r]);
         }
         nn[r] = MIXED ? a.nvars[v] : S;
+        xy[r] = (unsigned)(S | (S << 4));
+    }
+    if (any_slots) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            unsigned s = sb + r * WAVE + lane;
+            if (s >= se)
+                s = s_last;
+            if (MODE != PREP) {
+                u[r] = BP_NT_LOADS ? __builtin_nontemporal_load(a.slot_src + s) : a.slot_src[s];
+                if constexpr (MIXED)
+                    xy[r] = a.slot_xy[s];
+                if constexpr (POOL)
+                    tix[r] = a.slot_tab[s];
+                else if constexpr (HOLD)
+                    load_table<S, MIXED, PACKED, POOLED>(a, s, xy[r], J[r]);
+            } else if constexpr (PACKED) {
+                xy[r] = a.slot_xy[s];
+            }
+            if (!PACKED && (MODE == ITER || MODE == EDGE || MODE == PREP))
+                load_rowp_nt<S, SP>(a.msg_in + (size_t)s * SP, old[r]);
+        }
     }
     /* compact rows: each lane's message and node row offsets */
     unsigned mrow[R], nrow[R];
@@ -833,6 +875,11 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
             else
                 load_rowp<S, SP>(src + (size_t)u[r] * SP, bsrc[r]);
         }
+        if constexpr (POOL) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                load_pool_raw<S>(a.J + (size_t)(tix[r] & 0x7fffffffu) * S * S, praw[r]);
+        }
     }
     /* slot phase: gather, send, write, delta, stage in LDS */
 #pragma unroll
@@ -846,10 +893,12 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                     m[k] = old[r][k];
             } else {
                 unsigned x = xy[r] & 15u, y = xy[r] >> 4;
+                if constexpr (POOL)
+                    pool_table<S>(praw[r], (tix[r] >> 31) != 0, J[r]);
                 if constexpr (HOLD)
                     send<S, MIXED>(J[r], bsrc[r], x, y, m);
                 else
-                    send_from_memory<S, MIXED, PACKED>(a, s, xy[r], bsrc[r], m);
+                    send_from_memory<S, MIXED, PACKED, POOLED>(a, s, xy[r], bsrc[r], m);
                 if (MODE == ITER || MODE == EDGE) {
 #pragma unroll
                     for (int i = 0; i < S; ++i) {
@@ -895,7 +944,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
  * sum of the rows in row order, gathered by shuffles, so every operation is
  * the one send() performs, in its order.  Used for large state counts, where
  * one lane per slot would read a whole table alone. */
-template <int S, bool MIXED, int MODE, int L, bool PACKED>
+template <int S, bool MIXED, int MODE, int L, bool PACKED, bool POOLED>
 __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsigned ne, unsigned sb, unsigned se,
                                            float *lds, float *stage_f, unsigned *stage_u, unsigned lane, double &dacc,
                                            uint2 cbase)
@@ -918,7 +967,7 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
     if (slot_ok) {
         if (MIXED)
             xy = a.slot_xy[s];
-        if (PACKED || MIXED)
+        if (PACKED || MIXED || POOLED)
             t = a.slot_tab[s];
         if (MODE != PREP)
             u = a.slot_src[s];
@@ -1000,13 +1049,23 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
             x = xy & 15u;
             y = xy >> 4;
         }
-        const float *T = PACKED ? a.tabs + stage_u[W + q] : MIXED ? a.J + stage_u[W + q] : a.J + (size_t)s * S * S;
+        const float *T = PACKED ? a.tabs + stage_u[W + q]
+                         : MIXED ? a.J + stage_u[W + q]
+                         : POOLED ? a.J + (size_t)(stage_u[W + q] & 0x7fffffffu) * S * S
+                                  : a.J + (size_t)s * S * S;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             unsigned i = k + r * L;
             if (i < (unsigned)S && (!MIXED || i < x)) {
                 if constexpr (PACKED) {
                     load_pooled_row<S>(T, i, y, tv[r]);
+                } else if constexpr (POOLED) {
+                    /* row i of the slot's table: row i of the stored one,
+                     * or its column i when the slot reads it transposed */
+                    const bool tr = (stage_u[W + q] >> 31) != 0;
+#pragma unroll
+                    for (int j = 0; j < S; ++j)
+                        tv[r][j] = T[tr ? j * S + i : i * S + j];
                 } else if constexpr (MIXED) {
 #pragma unroll
                     for (int j = 0; j < S; ++j)
@@ -1132,7 +1191,7 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
  * for the belief update, then the combine buffer from the updated state --
  * recomputing the messages on the second walk instead of reading back what
  * other lanes wrote. */
-template <int S, bool MIXED, int MODE, bool PACKED>
+template <int S, bool MIXED, int MODE, bool PACKED, bool POOLED>
 __device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *lds, unsigned lane, double &dacc,
                           bool first, float *acc, unsigned n, unsigned mbase)
 {
@@ -1153,10 +1212,10 @@ __device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *l
         }
         if (s < se) {
             if (first) {
-                slot_message<S, MIXED, MODE, PACKED>(a, s, m, dacc, mrow);
+                slot_message<S, MIXED, MODE, PACKED, POOLED>(a, s, m, dacc, mrow);
             } else {
                 unsigned x;
-                compute_message<S, MIXED, MODE, PACKED>(a, s, m, x, mrow);
+                compute_message<S, MIXED, MODE, PACKED, POOLED>(a, s, m, x, mrow);
             }
 #pragma unroll
             for (int k = 0; k < S; ++k)
@@ -1176,7 +1235,7 @@ __device__ void walk_wide(const LoopyArgs &a, unsigned sb, unsigned se, float *l
     }
 }
 
-template <int S, bool MIXED, int MODE, bool PACKED>
+template <int S, bool MIXED, int MODE, bool PACKED, bool POOLED>
 __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned se, float *lds, unsigned lane,
                            double &dacc, uint2 cbase)
 {
@@ -1194,7 +1253,7 @@ __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned
 #pragma unroll
     for (int k = 0; k < S; ++k)
         acc[k] = (MODE == ITER || MODE == FWD) ? 1.0f : st[k];
-    walk_wide<S, MIXED, MODE, PACKED>(a, sb, se, lds, lane, dacc, true, acc, n, cbase.y);
+    walk_wide<S, MIXED, MODE, PACKED, POOLED>(a, sb, se, lds, lane, dacc, true, acc, n, cbase.y);
     if (MODE == ITER || MODE == FWD) {
 #pragma unroll
         for (int k = 0; k < S; ++k)
@@ -1204,7 +1263,7 @@ __device__ void chunk_wide(const LoopyArgs &a, unsigned v, unsigned sb, unsigned
 #pragma unroll
         for (int k = 0; k < S; ++k)
             acc[k] = st[k];
-        walk_wide<S, MIXED, MODE, PACKED>(a, sb, se, lds, lane, dacc, false, acc, n, cbase.y);
+        walk_wide<S, MIXED, MODE, PACKED, POOLED>(a, sb, se, lds, lane, dacc, false, acc, n, cbase.y);
     }
     if (MODE == EDGE)
         normalise_positive<S, MIXED>(acc, n);
@@ -1219,7 +1278,7 @@ template <int S> struct SlotsPerLane {
 
 /* One chunk: the narrow path, or the one-node path for a node with more
  * in-slots than a chunk holds. */
-template <int S, bool MIXED, int MODE, int R, bool PACKED, int L>
+template <int S, bool MIXED, int MODE, int R, bool PACKED, int L, bool POOLED>
 __device__ __forceinline__ void run_chunk(const LoopyArgs &a, unsigned c, float *lds, float *stage_f,
                                           unsigned *stage_u, unsigned lane, double &dacc)
 {
@@ -1229,11 +1288,11 @@ __device__ __forceinline__ void run_chunk(const LoopyArgs &a, unsigned c, float 
     if constexpr (PACKED)
         cbase = a.chunk_base[c];
     if (L > 1 && se - sb <= (unsigned)WAVE)
-        chunk_rows<S, MIXED, MODE, L, PACKED>(a, nb, ne, sb, se, lds, stage_f, stage_u, lane, dacc, cbase);
+        chunk_rows<S, MIXED, MODE, L, PACKED, POOLED>(a, nb, ne, sb, se, lds, stage_f, stage_u, lane, dacc, cbase);
     else if (L == 1 && se - sb <= (unsigned)(WAVE * R))
-        chunk_narrow<S, MIXED, MODE, R, PACKED>(a, nb, ne, sb, se, lds, lane, dacc, cbase);
+        chunk_narrow<S, MIXED, MODE, R, PACKED, POOLED>(a, nb, ne, sb, se, lds, lane, dacc, cbase);
     else if constexpr (!PACKED)
-        chunk_wide<S, MIXED, MODE, PACKED>(a, nb, sb, se, lds, lane, dacc, cbase);
+        chunk_wide<S, MIXED, MODE, PACKED, POOLED>(a, nb, sb, se, lds, lane, dacc, cbase);
     /* (PACKED: a node with more in-slots than a chunk holds runs in
      * k_loopy_wide after this launch, so the rare one-node path does not
      * set this kernel's register budget) */
@@ -1250,7 +1309,10 @@ __device__ __forceinline__ void flush_chunk_partial(const LoopyArgs &a, unsigned
     dacc = 0.0;
 }
 
-template <int S, bool MIXED, int MODE, int R = SlotsPerLane<S>::value, bool PACKED = false, int L = 1>
+/* POOLED: a uniform-state handle whose transposed pairs share one stored
+ * table (slot_tab[s] = pool entry, bit 31 = read transposed) */
+template <int S, bool MIXED, int MODE, int R = SlotsPerLane<S>::value, bool PACKED = false, int L = 1,
+          bool POOLED = false>
 __global__ __launch_bounds__(BLOCK, (MinWaves<PACKED, S, L>::value)) void k_loopy(LoopyArgs a)
 {
     if (a.stop && *a.stop)
@@ -1281,7 +1343,7 @@ __global__ __launch_bounds__(BLOCK, (MinWaves<PACKED, S, L>::value)) void k_loop
             unsigned end = base + a.qblock < a.nchunks ? base + a.qblock : a.nchunks;
             for (unsigned p = base; p < end; ++p) {
                 unsigned c = a.order ? a.order[p] : p;
-                run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds_w, stage_f[wave], stage_u[wave], lane, dacc);
+                run_chunk<S, MIXED, MODE, R, PACKED, L, POOLED>(a, c, lds_w, stage_f[wave], stage_u[wave], lane, dacc);
                 if (per_chunk)
                     flush_chunk_partial(a, c, lane, dacc);
             }
@@ -1296,7 +1358,7 @@ __global__ __launch_bounds__(BLOCK, (MinWaves<PACKED, S, L>::value)) void k_loop
         unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / W);
         for (unsigned p = c0; p < c1; ++p) {
             unsigned c = a.order ? a.order[p] : p;
-            run_chunk<S, MIXED, MODE, R, PACKED, L>(a, c, lds_w, stage_f[wave], stage_u[wave], lane, dacc);
+            run_chunk<S, MIXED, MODE, R, PACKED, L, POOLED>(a, c, lds_w, stage_f[wave], stage_u[wave], lane, dacc);
             if (per_chunk)
                 flush_chunk_partial(a, c, lane, dacc);
         }
@@ -1324,7 +1386,7 @@ __global__ __launch_bounds__(BLOCK) void k_loopy_wide(LoopyArgs a, const unsigne
     unsigned c = wide[i];
     uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
     double dacc = 0.0;
-    chunk_wide<S, MIXED, MODE, true>(a, lo.x, lo.y, hi.y, lds[wave], lane, dacc, a.chunk_base[c]);
+    chunk_wide<S, MIXED, MODE, true, false>(a, lo.x, lo.y, hi.y, lds[wave], lane, dacc, a.chunk_base[c]);
     wave_sync();
     if (MODE == ITER || MODE == EDGE)
         flush_chunk_partial(a, c, lane, dacc);
