@@ -144,6 +144,20 @@ struct LoopyArgs {
     unsigned qblock;
 };
 
+/* A wave-uniform read of data the launch does not write (chunk records,
+ * chunk order): through the constant address space, so the compiler reads
+ * it with one scalar load into SGPRs instead of a vector load per field. */
+__device__ __forceinline__ unsigned uniform_load(const unsigned *p)
+{
+    return *(const __attribute__((address_space(4))) unsigned *)p;
+}
+
+__device__ __forceinline__ uint2 uniform_load(const uint2 *p)
+{
+    unsigned long long v = *(const __attribute__((address_space(4))) unsigned long long *)p;
+    return make_uint2((unsigned)v, (unsigned)(v >> 32));
+}
+
 template <int S> struct Stride {
     static constexpr int value = (S <= 2) ? S : (S <= 4) ? 4 : (S <= 8) ? 8 : 10;
 };
@@ -1282,11 +1296,11 @@ template <int S, bool MIXED, int MODE, int R, bool PACKED, int L, bool POOLED>
 __device__ __forceinline__ void run_chunk(const LoopyArgs &a, unsigned c, float *lds, float *stage_f,
                                           unsigned *stage_u, unsigned lane, double &dacc)
 {
-    uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
+    uint2 lo = uniform_load(a.chunk + c), hi = uniform_load(a.chunk + c + 1);
     unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
     uint2 cbase = make_uint2(0u, 0u);
     if constexpr (PACKED)
-        cbase = a.chunk_base[c];
+        cbase = uniform_load(a.chunk_base + c);
     if (L > 1 && se - sb <= (unsigned)WAVE)
         chunk_rows<S, MIXED, MODE, L, PACKED, POOLED>(a, nb, ne, sb, se, lds, stage_f, stage_u, lane, dacc, cbase);
     else if (L == 1 && se - sb <= (unsigned)(WAVE * R))
@@ -1342,7 +1356,7 @@ __global__ __launch_bounds__(BLOCK, (MinWaves<PACKED, S, L>::value)) void k_loop
                 break;
             unsigned end = base + a.qblock < a.nchunks ? base + a.qblock : a.nchunks;
             for (unsigned p = base; p < end; ++p) {
-                unsigned c = a.order ? a.order[p] : p;
+                unsigned c = a.order ? uniform_load(a.order + p) : p;
                 run_chunk<S, MIXED, MODE, R, PACKED, L, POOLED>(a, c, lds_w, stage_f[wave], stage_u[wave], lane, dacc);
                 if (per_chunk)
                     flush_chunk_partial(a, c, lane, dacc);
@@ -1357,7 +1371,7 @@ __global__ __launch_bounds__(BLOCK, (MinWaves<PACKED, S, L>::value)) void k_loop
         unsigned c0 = (unsigned)(((unsigned long long)w * a.nchunks) / W);
         unsigned c1 = (unsigned)(((unsigned long long)(w + 1) * a.nchunks) / W);
         for (unsigned p = c0; p < c1; ++p) {
-            unsigned c = a.order ? a.order[p] : p;
+            unsigned c = a.order ? uniform_load(a.order + p) : p;
             run_chunk<S, MIXED, MODE, R, PACKED, L, POOLED>(a, c, lds_w, stage_f[wave], stage_u[wave], lane, dacc);
             if (per_chunk)
                 flush_chunk_partial(a, c, lane, dacc);
