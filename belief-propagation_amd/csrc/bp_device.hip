@@ -596,8 +596,11 @@ static int upload_state(bp_dev *d, Graph_t g, bool tables)
     d->cur_is_a = (g->current_edge_messages == &g->edges_messages);
     if (tables) {
         unsigned S = d->S;
-        /* mixed: x rows of y entries at toff[s]; uniform: [s][S*S], zeros
-         * beyond x rows and y columns */
+        /* mixed: x rows at toff[s], row stride mixed_row_stride(S), zeros
+         * beyond y; uniform: [s][S*S], zeros beyond x rows and y columns */
+        const unsigned RS = mixed_row_stride(S);
+        if (d->mixed)
+            memset(tmp + d->jfloats - (size_t)S * RS, 0, (size_t)S * RS * sizeof(float));
         par_for(d->E, [&](size_t a, size_t b) {
             for (size_t e = a; e < b; ++e) {
                 unsigned s = d->edge_slot[e];
@@ -606,8 +609,8 @@ static int upload_state(bp_dev *d, Graph_t g, bool tables)
                 if (d->mixed) {
                     float *T = tmp + d->toff[s];
                     for (unsigned i = 0; i < x; ++i)
-                        for (unsigned j = 0; j < y; ++j)
-                            T[i * y + j] = J[i * MAX_STATES + j];
+                        for (unsigned j = 0; j < RS; ++j)
+                            T[i * RS + j] = (j < y) ? J[i * MAX_STATES + j] : 0.0f;
                 } else {
                     float *T = tmp + (size_t)s * S * S;
                     for (unsigned i = 0; i < S; ++i)
@@ -841,16 +844,21 @@ static int upload_impl(Graph_t g, int device, bp_dev **out, bool plan)
     d->alg_bytes = algorithmic_bytes(g);
 
     size_t SP = d->SP;
-    /* mixed state counts: tables packed at their own sizes (a 2x2 table
-     * takes 4 floats, not S*S) behind per-slot offsets */
+    /* mixed state counts: each slot's table is its own x rows, each row
+     * padded to mixed_row_stride(S) floats (16-byte aligned rows, so a row
+     * is whole vector loads), behind a per-slot offset; the kernels read
+     * every row up to S unconditionally, so the array carries S rows of
+     * slack past the last table */
     if (d->mixed) {
+        const size_t RS = mixed_row_stride(S);
         d->toff.resize(E);
         size_t tot = 0;
         for (unsigned s = 0; s < E; ++s) {
             unsigned e = d->slot_edge[s];
             d->toff[s] = (unsigned)tot;
-            tot += (size_t)g->edges_x_dim[e] * g->edges_y_dim[e];
+            tot += (size_t)g->edges_x_dim[e] * RS;
         }
+        tot += (size_t)S * RS;
         if (tot >= 0xffffffffull) {
             bp_set_error("upload: more than 2^32 table entries");
             bp_free(d);

@@ -222,7 +222,7 @@ __global__ __launch_bounds__(BLOCK) void k_adjoint(AdjArgs a)
     float *LM = lm[wave], *LDT = ldt[wave];
     unsigned short *LN = lnode[wave], *LQ = lq[wave];
     for (unsigned c = c0; c < c1; ++c) {
-        uint2 lo = a.chunk[c], hi = a.chunk[c + 1];
+        uint2 lo = bpk::uniform_load(a.chunk + c), hi = bpk::uniform_load(a.chunk + c + 1);
         unsigned nb = lo.x, sb = lo.y, ne = hi.x, se = hi.y;
         /* slot phase 1: the message of every slot, kept with its operands */
         float Jr[R][S * S], b[R][S], m[R][S], sig[R];

@@ -10,7 +10,8 @@
  *   in_off[V+1]      slot range of each node
  *   slot_src[E]      source node of the slot's edge
  *   J[E][S*S]        the edge's table, J[i][j], i = source state (mixed
- *                    state counts: x rows of y floats at J + slot_tab[s];
+ *                    state counts: x rows of mixed_row_stride(S) floats at
+ *                    J + slot_tab[s];
  *                    batches: a pooled table at tabs + slot_tab[s]), j = destination state
  *   msg[2][E][SP]    messages, double-buffered (old / new)
  *   states[V][SP], buf[2][V][SP]  beliefs and the combine buffer
@@ -135,8 +136,9 @@ struct LoopyArgs {
      * tabs + slot_tab[s], row-major with the slot's x rows of y entries,
      * each row padded with zeros to a multiple of four floats (16-byte
      * aligned rows).  Otherwise, with mixed state counts, slot s's table is
-     * J + slot_tab[s], x rows of y entries with no padding (uniform state
-     * counts: J[s][S*S], slot_tab unused). */
+     * J + slot_tab[s], x rows of y entries, each row padded to
+     * mixed_row_stride(S) floats (uniform state counts: J[s][S*S], or with
+     * a shared pool slot_tab[s] = pool entry | transposed << 31). */
     const unsigned *slot_tab;
     const float *tabs;
     double *chunk_partials;      /* non-null: one delta partial per chunk instead of per block */
@@ -161,6 +163,11 @@ __device__ __forceinline__ uint2 uniform_load(const uint2 *p)
 template <int S> struct Stride {
     static constexpr int value = (S <= 2) ? S : (S <= 4) ? 4 : (S <= 8) ? 8 : 10;
 };
+
+/* Mixed state counts (plain handles): a slot's table is its x rows, each
+ * padded to this many floats (S = the handle's largest state count), so
+ * every row starts 16-byte aligned and is read as whole vector loads. */
+__host__ __device__ constexpr unsigned mixed_row_stride(unsigned S) { return (S + 3u) & ~3u; }
 
 /* The batch kernel for up to 8 states with 2 lanes per slot needs 90
  * registers, which leaves room for 5 waves per SIMD; held to 80 (6 waves,
@@ -300,6 +307,28 @@ __device__ __forceinline__ void load_pooled_row(const float *__restrict__ T, uns
     }
 }
 
+/* A mixed-state table (plain handles) at T into J[S][S]: its rows at
+ * stride mixed_row_stride(S), every row up to S read unconditionally as
+ * aligned four-float vectors (the table array carries slack past its last
+ * table).  No branch, no runtime index into registers.  Entries outside x
+ * rows and y columns hold whatever lies there: send() reads only i < x,
+ * j < y. */
+template <int S>
+__device__ __forceinline__ void load_mixed_rows(const float *__restrict__ T, float *J)
+{
+    constexpr int RS = (int)mixed_row_stride(S);
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+#pragma unroll
+        for (int q = 0; q < RS / 4; ++q) {
+            const v4f v = *reinterpret_cast<const v4f *>(T + i * RS + 4 * q);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (4 * q + c < S)
+                    J[i * S + 4 * q + c] = v[c];
+        }
+}
+
 /* A uniform-state table from the shared pool: entry slot_tab[s] of J,
  * read transposed when bit 31 is set (the slot is the later one of a pair
  * whose tables are exact transposes). */
@@ -358,13 +387,7 @@ __device__ __forceinline__ void load_table(const LoopyArgs &a, unsigned s, unsig
         else
             load_row_nt<S * S>(a.J + (size_t)s * S * S, J);
     } else if constexpr (!PACKED) {
-        unsigned x = xy & 15u, y = xy >> 4;
-        const float *T = a.J + a.slot_tab[s];
-#pragma unroll
-        for (int i = 0; i < S; ++i)
-#pragma unroll
-            for (int j = 0; j < S; ++j)
-                J[i * S + j] = ((unsigned)i < x && (unsigned)j < y) ? T[i * y + j] : 0.0f;
+        load_mixed_rows<S>(a.J + a.slot_tab[s], J);
     } else {
         unsigned x = xy & 15u, y = xy >> 4;
         const float *T = a.tabs + a.slot_tab[s];
@@ -402,9 +425,10 @@ __device__ __forceinline__ void send_from_memory(const LoopyArgs &a, unsigned s,
             if constexpr (PACKED) {
                 load_pooled_row<S>(T, i, y, row);
             } else if constexpr (MIXED) {
+                constexpr int RS = (int)mixed_row_stride(S);
 #pragma unroll
                 for (int j = 0; j < S; ++j)
-                    row[j] = (unsigned)j < y ? T[(size_t)i * y + j] : 0.0f;
+                    row[j] = (unsigned)j < y ? T[(size_t)i * RS + j] : 0.0f;
             } else {
 #pragma unroll
                 for (int j = 0; j < S; ++j)
@@ -809,7 +833,10 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
      * pool entry comes with the first loads and its table with the gathers,
      * transposed in registers when the message is formed */
     constexpr bool POOL = POOLED && HOLD && (S == 2 || S == 4);
-    unsigned u[R], xy[R], q0[R], q1[R], nn[R], tix[POOL ? R : 1];
+    /* mixed state counts: likewise the slot's table offset first, its rows
+     * (whole aligned vectors, no branch) with the gathers */
+    constexpr bool OFFT = MIXED && !PACKED && HOLD;
+    unsigned u[R], xy[R], q0[R], q1[R], nn[R], tix[(POOL || OFFT) ? R : 1];
     float J[HOLD ? R : 1][HOLD ? S * S : 1], old[R][S], st[R][S];
     PoolRaw<S> praw[POOL ? R : 1];
     /* Lanes past the chunk's end read a valid neighbour instead (and drop
@@ -849,7 +876,7 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 u[r] = BP_NT_LOADS ? __builtin_nontemporal_load(a.slot_src + s) : a.slot_src[s];
                 if constexpr (MIXED)
                     xy[r] = a.slot_xy[s];
-                if constexpr (POOL)
+                if constexpr (POOL || OFFT)
                     tix[r] = a.slot_tab[s];
                 else if constexpr (HOLD)
                     load_table<S, MIXED, PACKED, POOLED>(a, s, xy[r], J[r]);
@@ -893,6 +920,11 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
 #pragma unroll
             for (int r = 0; r < R; ++r)
                 load_pool_raw<S>(a.J + (size_t)(tix[r] & 0x7fffffffu) * S * S, praw[r]);
+        }
+        if constexpr (OFFT) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                load_mixed_rows<S>(a.J + tix[r], J[r]);
         }
     }
     /* slot phase: gather, send, write, delta, stage in LDS */
@@ -1081,9 +1113,10 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                     for (int j = 0; j < S; ++j)
                         tv[r][j] = T[tr ? j * S + i : i * S + j];
                 } else if constexpr (MIXED) {
+                    constexpr int RS = (int)mixed_row_stride(S);
 #pragma unroll
                     for (int j = 0; j < S; ++j)
-                        tv[r][j] = (unsigned)j < y ? T[(size_t)i * y + j] : 0.0f;
+                        tv[r][j] = (unsigned)j < y ? T[(size_t)i * RS + j] : 0.0f;
                 } else {
 #pragma unroll
                     for (int j = 0; j < S; ++j)

@@ -71,7 +71,7 @@ struct TreeEntry {
 struct TreeArgs {
     const unsigned char *slot_xy;
     const float *J;
-    const unsigned *slot_tab; /* mixed state counts: table offsets in J (x rows of y, no padding) */
+    const unsigned *slot_tab; /* mixed state counts: table offsets in J (x rows, stride mixed_row_stride(S)) */
     const float *states;
     float *msg;
     const TreeEntry *ent;
@@ -128,12 +128,7 @@ __device__ __forceinline__ void tree_entry(const TreeArgs &a, unsigned entry, in
         }
         if constexpr (MIXED) {
             float Jd[S * S];
-            const float *T = a.J + a.slot_tab[s];
-#pragma unroll
-            for (int i = 0; i < S; ++i)
-#pragma unroll
-                for (int j = 0; j < S; ++j)
-                    Jd[i * S + j] = ((unsigned)i < x && (unsigned)j < y) ? T[i * y + j] : 0.0f;
+            load_mixed_rows<S>(a.J + a.slot_tab[s], Jd);
             send<S, MIXED>(Jd, buf, x, y, m);
         } else {
             send<S, MIXED>(a.J + (size_t)s * S * S, buf, x, y, m);
