@@ -399,12 +399,18 @@ extern "C" int bp_batch_add(bp_batch *b, Graph_t g, unsigned int *index)
  * together: every order gives the same results. */
 constexpr unsigned MAX_GROUP = 16;
 
+/* Most slots (and nodes) in a chunk of one slot per lane; a measuring
+ * switch (with two lanes per slot, a chunk of 32 slots takes one pass) */
+#ifndef BP_BATCH_CHUNK_CAP
+#define BP_BATCH_CHUNK_CAP 128
+#endif
+
 static void build_shape(bp_batch *b, Shape &sh, unsigned R)
 {
     sh.R = R;
     sh.chunks.clear();
     sh.inst_chunk.assign(1, 0);
-    unsigned cap = WAVE * R;
+    unsigned cap = std::min<unsigned>(WAVE * R, BP_BATCH_CHUNK_CAP);
     unsigned ninst = (unsigned)b->node_off.size() - 1;
     for (unsigned i = 0; i < ninst; ++i) {
         unsigned v = b->node_off[i], vend = b->node_off[i + 1];
@@ -438,7 +444,7 @@ static void build_shape(bp_batch *b, Shape &sh, unsigned R)
     }
     sh.wide.clear();
     for (unsigned c = 0; c < sh.nchunks; ++c)
-        if (sh.chunks[c + 1].y - sh.chunks[c].y > cap)
+        if (sh.chunks[c + 1].y - sh.chunks[c].y > WAVE * R) /* beyond what the main kernel takes */
             sh.wide.push_back(c);
     sh.order.clear();
     sh.order.reserve(sh.nchunks);

@@ -1207,6 +1207,8 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
     };
 #pragma unroll 1
     for (int p = 0; p < L; ++p) {
+        if ((unsigned)(p * G) >= se - sb)
+            break; /* the chunk's slots are all done */
         float tv[NR][S];
         load_pass(p, tv);
         run_pass(p, tv);
