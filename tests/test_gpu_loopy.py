@@ -236,6 +236,91 @@ def test_high_in_degree_matches_oracle(lib, orc, n_leaves, states):
     lib.graph_destroy(h)
 
 
+def _pair_graph(lib, n_nodes: int, pairs, states, seed: int):
+    """Nodes with states[v % len(states)] states and, for every (a, b) in
+    pairs, an edge a -> b and its reverse b -> a carrying the transposed
+    table, as the parsers emit them (a handle may then share one stored
+    table per pair)."""
+    import ctypes as C
+
+    rng = np.random.default_rng(seed)
+    g = lib.create_graph(n_nodes, 2 * len(pairs))
+    ns = [states[v % len(states)] for v in range(n_nodes)]
+    st = (C.c_float * 10)()
+    for v in range(n_nodes):
+        lib.graph_add_node(g, ns[v], f"n{v}".encode())
+        if v % 3 == 0:
+            p = rng.random(ns[v]).astype(np.float32)
+            for k in range(ns[v]):
+                st[k] = float(p[k])
+            lib.graph_set_node_state(g, v, ns[v], st)
+    tab = (C.c_float * 100)()
+    for a_, b_ in pairs:
+        J = rng.random((ns[a_], ns[b_])).astype(np.float32)
+        for i in range(ns[a_]):
+            for j in range(ns[b_]):
+                tab[i * 10 + j] = float(J[i, j])
+        lib.graph_add_edge(g, a_, b_, ns[a_], ns[b_], tab)
+        for i in range(ns[b_]):
+            for j in range(ns[a_]):
+                tab[i * 10 + j] = float(J[j, i])
+        lib.graph_add_edge(g, b_, a_, ns[b_], ns[a_], tab)
+    lib.set_up_src_nodes_to_edges(g)
+    lib.set_up_dest_nodes_to_edges(g)
+    return g
+
+
+def _lattice_with_hub(side: int, hub_leaves: int):
+    """A side x side lattice (node 0 is an extra hub) whose hub is joined to
+    the first hub_leaves lattice nodes: nearly every pair is local, and the
+    hub has more in-slots than a chunk holds."""
+    pairs = []
+    for r in range(side):
+        for c in range(side):
+            v = 1 + r * side + c
+            if c + 1 < side:
+                pairs.append((v, v + 1))
+            if r + 1 < side:
+                pairs.append((v, v + side))
+    # the hub's neighbours are consecutive lattice nodes: a hub joined to
+    # nodes far apart would be a shortcut that scatters the device order
+    pairs += [(1 + k, 0) for k in range(hub_leaves)]
+    return 1 + side * side, pairs
+
+
+@pytest.mark.parametrize("shape,states,shared,mixed", [
+    (("lattice", 128, 150), (2,), 1, 0), (("lattice", 128, 150), (4,), 1, 0),
+    (("star", 300), (4, 2, 3), 0, 1), (("star", 90), (3, 2, 3), 0, 1)])
+def test_high_in_degree_pairs_match_oracle(lib, orc, shape, states, shared, mixed):
+    """A node above the chunk cap on a handle that shares one stored table
+    per transposed pair (the shared-pool kernels: the later slot of each
+    pair reads its partner's table transposed, the hub through the one-node
+    path), and with mixed state counts (tables of padded rows, read whole by
+    the narrow path and row by row by the one-node path)."""
+    print(f"pairs {shape} states {states}", flush=True)
+    if shape[0] == "lattice":
+        n, pairs = _lattice_with_hub(shape[1], shape[2])
+    else:
+        n, pairs = shape[1] + 1, [(k + 1, 0) for k in range(shape[1])]
+    g = _pair_graph(lib, n, pairs, states, seed=n)
+    h = _pair_graph(lib, n, pairs, states, seed=n)
+    dev = B.Device(g)
+    info = dev.info()
+    assert info["shared_tables"] == shared and info["mixed"] == mixed, info
+    dev.init()
+    orc.init_previous_edge(h)
+    dev.iterate(6)
+    for _ in range(6):
+        orc.loopy_iteration(h)
+    dev.download(g)
+    assert same_bits(B.states(g), B.states(h)), bits_report(B.states(g), B.states(h))
+    assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
+    assert same_bits(B.messages(g, "cur"), B.messages(h, "cur"))
+    dev.free()
+    lib.graph_destroy(g)
+    lib.graph_destroy(h)
+
+
 def test_send_message_dropin(lib, orc):
     """send_message (graph.c:426-448), one call per edge over every edge of
     the lattice, run on the GPU, each equal to the oracle's send."""
