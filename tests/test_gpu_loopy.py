@@ -316,9 +316,32 @@ def test_high_in_degree_pairs_match_oracle(lib, orc, shape, states, shared, mixe
     assert same_bits(B.states(g), B.states(h)), bits_report(B.states(g), B.states(h))
     assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
     assert same_bits(B.messages(g, "cur"), B.messages(h, "cur"))
+    # the host's arrays back on the device: the combine buffer is rebuilt
+    # from them (PREP) before two more iterations
+    B.check(lib.bp_upload_state(dev.h, g), "bp_upload_state")
+    dev.iterate(2)
+    for _ in range(2):
+        orc.loopy_iteration(h)
+    dev.download(g)
+    assert same_bits(B.states(g), B.states(h)), bits_report(B.states(g), B.states(h))
+    assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
     dev.free()
-    lib.graph_destroy(g)
-    lib.graph_destroy(h)
+    # the edge-centric schedule on the same handle kind
+    g2 = _pair_graph(lib, n, pairs, states, seed=n)
+    h2 = _pair_graph(lib, n, pairs, states, seed=n)
+    dev = B.Device(g2)
+    dev.init()
+    B.check(lib.bp_edge_iterate(dev.h, 4), "bp_edge_iterate")
+    orc.init_previous_edge(h2)
+    for _ in range(4):
+        orc.edge_iteration(h2)
+    dev.download(g2)
+    assert same_bits(B.states(g2), B.states(h2)), bits_report(B.states(g2), B.states(h2))
+    for which in ("cur", "prev"):
+        assert same_bits(B.messages(g2, which), B.messages(h2, which)), which
+    dev.free()
+    for x in (g, h, g2, h2):
+        lib.graph_destroy(x)
 
 
 def test_send_message_dropin(lib, orc):
