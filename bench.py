@@ -131,7 +131,7 @@ def bench_adjoint(K: int = 30, c: int = 5, reps: int = 10) -> dict:
     return out
 
 
-FORMS = ("adjoint", "batched", "tree")
+FORMS = ("adjoint", "batched", "tree", "configs")
 
 def reference_cpu(bn, threads: int, gpu_states) -> dict | None:
     """The reference's own node loop (oracle/_ref/ref_harness_omp, compiled
@@ -183,6 +183,56 @@ PRESET_KIND = {
     "c3": "seeded 1024x1024 lattice, 4 states",
     "c5": "seeded 200k-node DAG, 3 states",
 }
+
+
+def bench_configs(presets=("c2", "c3"), steps: int = 20, warmup: int = 3) -> dict:
+    """The other plain loopy workloads, timed the way the headline is: a step
+    is restore + init_previous_edge + ITERATIONS iterations (each with its
+    FP64 delta), and the iteration kernel is timed with HIP events at its own
+    start and end, mean of ITERATIONS launches.  The lattice config runs 50
+    iterations in its parity tests; its per-iteration throughput is the same
+    at 20.  Parity for these workloads is covered by the GPU tests."""
+    lib = B.load()
+    out = {}
+    for p in presets:
+        bn = B.generate(p)
+        g = B.bn_to_graph(bn)
+        lib.bp_bn_free(bn)
+        E = g.contents.current_num_edges
+        dev = B.Device(g, 0)
+        info = dev.info()
+
+        def step():
+            dev.restore()
+            dev.init()
+            dev.iterate(ITERATIONS)
+
+        for _ in range(warmup):
+            step()
+        dev.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        dev.sync()
+        elapsed = time.perf_counter() - t0
+        dev.restore()
+        dev.init()
+        kernel_us = float(np.mean(dev.profile(ITERATIONS))) * 1e3
+        alg = dev.algorithmic_bytes()
+        achieved = alg / (kernel_us * 1e-6) / 1e9
+        rocprof_us = read_profile_value(f"loopy_iter_{p}_kernel_trace.md", "iteration_kernel_avg_us")
+        out[p] = {"workload": f"{p}: {PRESET_KIND.get(p, 'seeded network')}, init + {ITERATIONS} loopy iterations",
+                  "message_edges": E, "mixed_states": bool(info["mixed"]),
+                  "shared_pair_tables": bool(info["shared_tables"]),
+                  "ms_per_step": 1e3 * elapsed / steps, "msg_updates_per_s": E * ITERATIONS * steps / elapsed,
+                  "kernel_us": kernel_us, "algorithmic_bytes_per_launch": alg, "achieved_GBs": achieved,
+                  "frac": achieved / HBM_PEAK_GBS,
+                  "traffic_bytes": read_traffic(f"loopy_iter_{p}_traffic.txt"),
+                  "rocprof_kernel_us": rocprof_us,
+                  "frac_rocprof": (alg / (rocprof_us * 1e-6) / 1e9 / HBM_PEAK_GBS) if rocprof_us else None}
+        dev.free()
+        lib.graph_destroy(g)
+    return out
 
 
 def bench_tree(reps: int = 7) -> dict:
@@ -303,9 +353,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--preset", default="c2b")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--forms", default="adjoint,batched,tree",
-                    help="comma list of the further forms measured in the same run (adjoint, batched, tree; '' "
-                         "for none)")
+    ap.add_argument("--forms", default="adjoint,batched,tree,configs",
+                    help="comma list of the further forms measured in the same run (adjoint, batched, tree, "
+                         "configs: the mixed-state DAG and the lattice timed like the headline; '' for none)")
     ap.add_argument("--batch-nets", type=int, default=512, help="networks in the batched form (8 evidence sets each)")
     ap.add_argument("--no-batch-plain", action="store_true",
                     help="batched form without its plain per-network comparison")
@@ -463,6 +513,9 @@ def main():
     if "tree" in wanted:
         progress("tree form")
         forms["tree"] = bench_tree()
+    if "configs" in wanted:
+        progress("plain configs: mixed-state DAG, lattice")
+        forms["configs"] = bench_configs(tuple(p for p in ("c2", "c3") if p != args.preset))
     if "batched" in wanted:
         forms["batched"] = bench_batched(count=args.batch_nets, plain_leg=not args.no_batch_plain)
 
