@@ -526,6 +526,11 @@ extern "C" int bp_batch_upload(bp_batch *b, int device)
             r += (b->nvars[v] + 1u) & ~1u;
         }
     }
+    /* zeros past the pool's last table: a lane reads every row of its pass
+     * (row indices below 16 = lanes per slot x rows per lane, at the
+     * table's own stride of at most 12 floats) as whole four-float vectors
+     * (at most 12 floats), whatever the table's row count */
+    b->tabs.resize(b->tabs.size() + (size_t)16 * pooled_stride(MAX_STATES), 0.0f);
     std::vector<unsigned> in_off(b->in_off);
     in_off.push_back(b->E);
     unsigned maxchunks = std::max(b->shapes[0].nchunks, b->shapes[1].nchunks);

@@ -502,16 +502,22 @@ __device__ __forceinline__ void load_crow_z(const float *__restrict__ p, unsigne
         out[k] = ((unsigned)k < n) ? out[k] : 0.0f;
 }
 
-/* the compact row of n entries v[0..n), zero-padded to an even count */
+/* the compact row of n entries v[0..n), zero-padded to an even count.
+ * Each pair's second entry is selected before, and outside, the branch
+ * that guards its store, and the empty asm keeps it there: with the select
+ * inside the branch, the optimiser merges the two tests on n into a switch
+ * (n = 0: no store, n = 1: second entry 0, else v[1]), and this compiler
+ * (ROCm 7.2, gfx950) can lower that divergent switch so that the else
+ * case stores 0 as well; measured on a one-wave test of the node phase. */
 template <int S>
 __device__ __forceinline__ void store_crow(float *__restrict__ p, unsigned n, const float *v)
 {
 #pragma unroll
     for (int q = 0; q < (S + 1) / 2; ++q) {
-        if ((unsigned)(2 * q) < n) {
-            float hi = (2 * q + 1 < S && (unsigned)(2 * q + 1) < n) ? v[2 * q + 1] : 0.0f;
+        float hi = (2 * q + 1 < S && (unsigned)(2 * q + 1) < n) ? v[2 * q + 1] : 0.0f;
+        asm volatile("" : "+v"(hi));
+        if ((unsigned)(2 * q) < n)
             *reinterpret_cast<float2 *>(p + 2 * q) = make_float2(v[2 * q], hi);
-        }
     }
 }
 
@@ -733,7 +739,10 @@ __device__ __forceinline__ void load_row(const float *__restrict__ p, float *out
 /* Node phase for node v with in-slots [q0, q1) of the chunk's LDS staging
  * (messages at lds[k * W + q]): the belief update of graph.c:985-1024
  * (ITER, FWD), the combine buffer of graph.c:851-882 for the next
- * iteration, or the edge schedule's product (EDGE). */
+ * iteration, or the edge schedule's product (EDGE).  The products run over
+ * all S entries: with mixed state counts, entries past the node's count n
+ * hold whatever the products give and are never used (the sums below and
+ * write_node take k < n only), which spares a mask per entry. */
 template <int S, bool MIXED, int MODE, bool PACKED = false>
 __device__ __forceinline__ void node_update(const LoopyArgs &a, size_t row, unsigned q0, unsigned q1, float *st,
                                             unsigned n, const float *lds, int W)
@@ -758,18 +767,17 @@ __device__ __forceinline__ void node_update(const LoopyArgs &a, size_t row, unsi
 #pragma unroll
         for (int j = 0; j < K; ++j)
             if ((unsigned)j < cnt)
-                combine<S, MIXED>(t, mk[j], n);
+                combine<S, false>(t, mk[j], n);
         for (unsigned q = q0 + K; q < q1; ++q) {
 #pragma unroll
             for (int k = 0; k < S; ++k)
                 m[k] = lds[k * W + q];
-            combine<S, MIXED>(t, m, n);
+            combine<S, false>(t, m, n);
         }
         if (q1 > q0) {
 #pragma unroll
             for (int k = 0; k < S; ++k)
-                if (!MIXED || (unsigned)k < n)
-                    st[k] = st[k] * t[k];
+                st[k] = st[k] * t[k];
         }
         normalise<S, MIXED>(st, n);
     }
@@ -779,16 +787,110 @@ __device__ __forceinline__ void node_update(const LoopyArgs &a, size_t row, unsi
 #pragma unroll
     for (int j = 0; j < K; ++j)
         if ((unsigned)j < cnt)
-            combine_mode<S, MIXED, MODE>(b, mk[j], n);
+            combine_mode<S, false, MODE>(b, mk[j], n);
     for (unsigned q = q0 + K; q < q1; ++q) {
 #pragma unroll
         for (int k = 0; k < S; ++k)
             m[k] = lds[k * W + q];
-        combine_mode<S, MIXED, MODE>(b, m, n);
+        combine_mode<S, false, MODE>(b, m, n);
     }
     if (MODE == EDGE)
         normalise_positive<S, MIXED>(b, n);
     write_node<S, MIXED, MODE, PACKED>(a, row, st, b, n);
+}
+
+/* Swap of a value between neighbouring lanes (2m <-> 2m + 1), by DPP. */
+__device__ __forceinline__ float pair_swap(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+}
+
+/* The node phase of node_update (ITER, INIT; compact rows) with two lanes
+ * per node, for a chunk of at most 32 nodes: lanes 2m and 2m + 1 hold
+ * node m's entries [0, H) and [H, 2H), H = S / 2, so each lane does half
+ * the products and divisions.  The node's fields come from lane m (which
+ * loaded them, one lane per node).  The normalising sum keeps the order of
+ * the one-lane form: the low lane adds its entries to 0, the high lane
+ * continues from that partial, and the total goes back to the low lane.
+ * Entries past the node's count n are never summed or stored. */
+template <int S, int MODE>
+__device__ __forceinline__ void node_update_pair(const LoopyArgs &a, unsigned nrow, unsigned q0, unsigned q1, unsigned n,
+                                                 bool ok, const float *lds, unsigned lane)
+{
+    constexpr int H = S / 2, K = 4;
+    const int src = (int)(lane >> 1);
+    const unsigned h = lane & 1u, k0 = h * H;
+    const unsigned row = (unsigned)__shfl((int)nrow, src, WAVE) + k0;
+    const unsigned a0 = (unsigned)__shfl((int)q0, src, WAVE), a1 = (unsigned)__shfl((int)q1, src, WAVE);
+    const unsigned nn = (unsigned)__shfl((int)n, src, WAVE);
+    if (!ok)
+        return;
+    const unsigned cnt = a1 - a0;
+    float mk[K][H], m[H], st[H], b[H];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if ((unsigned)j < cnt)
+#pragma unroll
+            for (int k = 0; k < H; ++k)
+                mk[j][k] = lds[(k0 + k) * WAVE + a0 + j];
+    load_crow<H>(a.states + row, st);
+    if (MODE == ITER) {
+        float t[H];
+#pragma unroll
+        for (int k = 0; k < H; ++k)
+            t[k] = 1.0f;
+#pragma unroll
+        for (int j = 0; j < K; ++j)
+            if ((unsigned)j < cnt)
+                combine<H, false>(t, mk[j], 0u);
+        for (unsigned q = a0 + K; q < a1; ++q) {
+#pragma unroll
+            for (int k = 0; k < H; ++k)
+                m[k] = lds[(k0 + k) * WAVE + q];
+            combine<H, false>(t, m, 0u);
+        }
+        if (a1 > a0) {
+#pragma unroll
+            for (int k = 0; k < H; ++k)
+                st[k] = st[k] * t[k];
+        }
+        float p = 0.0f;
+#pragma unroll
+        for (int k = 0; k < H; ++k)
+            if (h == 0 && (unsigned)k < nn)
+                p = p + st[k];
+        const float lo = pair_swap(p);
+        float sum = h ? lo : p;
+#pragma unroll
+        for (int k = 0; k < H; ++k)
+            if (h == 1 && (unsigned)(H + k) < nn)
+                sum = sum + st[k];
+        const float hi = pair_swap(sum);
+        sum = h ? sum : hi;
+        if (sum <= 0.0f)
+            sum = 1.0f;
+#pragma unroll
+        for (int k = 0; k < H; ++k)
+            if (k0 + k < nn)
+                st[k] = st[k] / sum;
+    }
+#pragma unroll
+    for (int k = 0; k < H; ++k)
+        b[k] = st[k];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if ((unsigned)j < cnt)
+            combine<H, false>(b, mk[j], 0u);
+    for (unsigned q = a0 + K; q < a1; ++q) {
+#pragma unroll
+        for (int k = 0; k < H; ++k)
+            m[k] = lds[(k0 + k) * WAVE + q];
+        combine<H, false>(b, m, 0u);
+    }
+    const unsigned own = nn > k0 ? nn - k0 : 0u;
+    if (MODE == ITER)
+        store_crow<H>(a.states + row, own, st);
+    store_crow<H>(a.buf_out + row, own, b);
 }
 
 /* The first S floats of a row of SP: one 16-byte load when a 3-state row
@@ -1082,6 +1184,34 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
      * costing occupancy). */
     auto load_pass = [&](int p, float (&tv)[NR][S]) {
         unsigned q = p * G + g, s = sb + q;
+        if constexpr (PACKED && MODE != PREP) {
+            /* Every row this lane multiplies, read unconditionally as
+             * aligned four-float vectors: row i of the pooled table starts
+             * at i * st, st = y rounded up to four, and the pool ends with
+             * slack for the rows and vectors past a table's last one.
+             * Columns y..st-1 are the pool's zero padding; a vector wholly
+             * past st belongs to the next row and is replaced by zeros, so
+             * every entry j >= y is +0.  Rows i >= x hold whatever lies
+             * there and are never used (run_pass drops them).  A slot past
+             * the chunk reads from the pool's start. */
+            const bool act = s < se;
+            const unsigned y = act ? (stage_u[q] >> 4) : 1u;
+            const unsigned st = (y + 3u) & ~3u;
+            const float *T = a.tabs + (act ? stage_u[W + q] : 0u) + k * st;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const float *row = T + (size_t)(r * L) * st;
+#pragma unroll
+                for (int qv = 0; qv < (S + 3) / 4; ++qv) {
+                    v4f w = *reinterpret_cast<const v4f *>(row + 4 * qv);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if (4 * qv + c < S)
+                            tv[r][4 * qv + c] = (qv == 0 || (unsigned)(4 * qv) < y) ? w[c] : 0.0f;
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -1146,6 +1276,25 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                     if (i < (unsigned)S)
                         acc[r] = stage_f[i * W + q];
                 }
+            } else if constexpr (PACKED) {
+                /* The staged row with its entries j >= y set to +0; every
+                 * table entry j >= y is +0 too (load_pass), so each such
+                 * product is +-0 and adding it leaves the sum unchanged (a
+                 * sum that starts at +0 is never -0): the unmasked sums
+                 * equal the sums over j < y bit for bit. */
+                float b[S];
+#pragma unroll
+                for (int j = 0; j < S; ++j)
+                    b[j] = (unsigned)j < y ? stage_f[j * W + q] : 0.0f;
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    unsigned i = k + r * L;
+                    float sacc = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < S; ++j)
+                        sacc = sacc + tv[r][j] * b[j];
+                    acc[r] = i < x ? sacc : 0.0f;
+                }
             } else {
                 float b[S];
 #pragma unroll
@@ -1173,11 +1322,23 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
         } else {
             /* the row sums in row order, from the lanes that hold them */
             float sum = 0.0f;
+            if constexpr (PACKED && L == 2) {
+                /* rows 2r and 2r + 1 sit in acc[r] of the pair's two
+                 * lanes: one swap of neighbouring lanes (DPP) per r.  Rows
+                 * past x hold +0, which leaves the sum unchanged. */
 #pragma unroll
-            for (int i = 0; i < S; ++i) {
-                float vi = __shfl(acc[i / L], (int)(base + (unsigned)(i % L)), WAVE);
-                if (!MIXED || (unsigned)i < x)
-                    sum = sum + vi;
+                for (int r = 0; r < NR; ++r) {
+                    const float o = pair_swap(acc[r]);
+                    sum = sum + (k ? o : acc[r]);
+                    sum = sum + (k ? acc[r] : o);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < S; ++i) {
+                    float vi = __shfl(acc[i / L], (int)(base + (unsigned)(i % L)), WAVE);
+                    if (!MIXED || (unsigned)i < x)
+                        sum = sum + vi;
+                }
             }
             if (sum <= 0.0f)
                 sum = 1.0f;
@@ -1227,6 +1388,12 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                 store_crow<S>(a.msg_out + mrow, stage_u[lane] & 15u, row);
             else
                 store_row_nt<SP>(a.msg_out + (size_t)s * SP, row);
+        }
+    }
+    if constexpr (PACKED && L == 2 && S == 8 && late_states && (MODE == ITER || MODE == INIT)) {
+        if (ne - nb <= (unsigned)(W / 2)) {
+            node_update_pair<S, MODE>(a, nrow, q0, q1, n, nb + (lane >> 1) < ne, lds, lane);
+            return;
         }
     }
     if (late_states)
