@@ -736,6 +736,25 @@ __device__ __forceinline__ void load_row(const float *__restrict__ p, float *out
     }
 }
 
+/* A message's NaN entries replaced by 1, once: multiplying by every entry
+ * of the result is then the NaN-skipping product of combine() (x * 1 == x
+ * for every x). */
+template <int S>
+__device__ __forceinline__ void nan_to_one(float *m)
+{
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        m[k] = (m[k] == m[k]) ? m[k] : 1.0f;
+}
+
+template <int S>
+__device__ __forceinline__ void mul_all(float *buf, const float *m)
+{
+#pragma unroll
+    for (int k = 0; k < S; ++k)
+        buf[k] = buf[k] * m[k];
+}
+
 /* Node phase for node v with in-slots [q0, q1) of the chunk's LDS staging
  * (messages at lds[k * W + q]): the belief update of graph.c:985-1024
  * (ITER, FWD), the combine buffer of graph.c:851-882 for the next
@@ -749,16 +768,22 @@ __device__ __forceinline__ void node_update(const LoopyArgs &a, size_t row, unsi
 {
     /* the first K in-messages are read from LDS once, all together, and
      * kept for both products (most nodes have no more); the rest are read
-     * in slot order as they are combined */
+     * in slot order as they are combined.  Except for the edge schedule
+     * (whose product takes NaN factors too), each message's NaN entries
+     * are replaced by 1 once, and both products multiply by every entry. */
     constexpr int K = (S <= 4) ? 4 : 2;
+    constexpr bool CLEAN = MODE != EDGE;
     const unsigned cnt = q1 - q0;
     float mk[K][S], m[S], b[S];
 #pragma unroll
     for (int j = 0; j < K; ++j)
-        if ((unsigned)j < cnt)
+        if ((unsigned)j < cnt) {
 #pragma unroll
             for (int k = 0; k < S; ++k)
                 mk[j][k] = lds[k * W + q0 + j];
+            if (CLEAN)
+                nan_to_one<S>(mk[j]);
+        }
     if (MODE == ITER || MODE == FWD) {
         float t[S];
 #pragma unroll
@@ -767,12 +792,13 @@ __device__ __forceinline__ void node_update(const LoopyArgs &a, size_t row, unsi
 #pragma unroll
         for (int j = 0; j < K; ++j)
             if ((unsigned)j < cnt)
-                combine<S, false>(t, mk[j], n);
+                mul_all<S>(t, mk[j]);
         for (unsigned q = q0 + K; q < q1; ++q) {
 #pragma unroll
             for (int k = 0; k < S; ++k)
                 m[k] = lds[k * W + q];
-            combine<S, false>(t, m, n);
+            nan_to_one<S>(m);
+            mul_all<S>(t, m);
         }
         if (q1 > q0) {
 #pragma unroll
@@ -786,13 +812,22 @@ __device__ __forceinline__ void node_update(const LoopyArgs &a, size_t row, unsi
         b[k] = st[k];
 #pragma unroll
     for (int j = 0; j < K; ++j)
-        if ((unsigned)j < cnt)
-            combine_mode<S, false, MODE>(b, mk[j], n);
+        if ((unsigned)j < cnt) {
+            if (CLEAN)
+                mul_all<S>(b, mk[j]);
+            else
+                combine_all<S, false>(b, mk[j], n);
+        }
     for (unsigned q = q0 + K; q < q1; ++q) {
 #pragma unroll
         for (int k = 0; k < S; ++k)
             m[k] = lds[k * W + q];
-        combine_mode<S, false, MODE>(b, m, n);
+        if (CLEAN) {
+            nan_to_one<S>(m);
+            mul_all<S>(b, m);
+        } else {
+            combine_all<S, false>(b, m, n);
+        }
     }
     if (MODE == EDGE)
         normalise_positive<S, MIXED>(b, n);
@@ -833,6 +868,10 @@ __device__ __forceinline__ void node_update_pair(const LoopyArgs &a, unsigned nr
 #pragma unroll
             for (int k = 0; k < H; ++k)
                 mk[j][k] = lds[(k0 + k) * WAVE + a0 + j];
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+        if ((unsigned)j < cnt)
+            nan_to_one<H>(mk[j]);
     load_crow<H>(a.states + row, st);
     if (MODE == ITER) {
         float t[H];
@@ -842,12 +881,13 @@ __device__ __forceinline__ void node_update_pair(const LoopyArgs &a, unsigned nr
 #pragma unroll
         for (int j = 0; j < K; ++j)
             if ((unsigned)j < cnt)
-                combine<H, false>(t, mk[j], 0u);
+                mul_all<H>(t, mk[j]);
         for (unsigned q = a0 + K; q < a1; ++q) {
 #pragma unroll
             for (int k = 0; k < H; ++k)
                 m[k] = lds[(k0 + k) * WAVE + q];
-            combine<H, false>(t, m, 0u);
+            nan_to_one<H>(m);
+            mul_all<H>(t, m);
         }
         if (a1 > a0) {
 #pragma unroll
@@ -880,12 +920,13 @@ __device__ __forceinline__ void node_update_pair(const LoopyArgs &a, unsigned nr
 #pragma unroll
     for (int j = 0; j < K; ++j)
         if ((unsigned)j < cnt)
-            combine<H, false>(b, mk[j], 0u);
+            mul_all<H>(b, mk[j]);
     for (unsigned q = a0 + K; q < a1; ++q) {
 #pragma unroll
         for (int k = 0; k < H; ++k)
             m[k] = lds[(k0 + k) * WAVE + q];
-        combine<H, false>(b, m, 0u);
+        nan_to_one<H>(m);
+        mul_all<H>(b, m);
     }
     const unsigned own = nn > k0 ? nn - k0 : 0u;
     if (MODE == ITER)
@@ -1140,7 +1181,10 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
         } else {
             const float *src = (MODE == INIT || MODE == EDGE || MODE == FINIT) ? a.states : a.buf_in;
             if constexpr (PACKED)
-                load_crow_z<S>(src + u, xy & 15u, b); /* the source's row: x entries */
+                /* the source's row: x entries, zero beyond (and, as the
+                 * products read only j < y, zero from y on as well, so
+                 * run_pass needs no mask of its own) */
+                load_crow_z<S>(src + u, min(xy & 15u, xy >> 4), b);
             else
                 load_rowp<S, SP>(src + (size_t)u * SP, b);
         }
@@ -1195,9 +1239,10 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
              * there and are never used (run_pass drops them).  A slot past
              * the chunk reads from the pool's start. */
             const bool act = s < se;
-            const unsigned y = act ? (stage_u[q] >> 4) : 1u;
+            const unsigned xy = stage_u[q], tq = stage_u[W + q]; /* stale past the chunk */
+            const unsigned y = act ? (xy >> 4) : 1u;
             const unsigned st = (y + 3u) & ~3u;
-            const float *T = a.tabs + (act ? stage_u[W + q] : 0u) + k * st;
+            const float *T = a.tabs + (act ? tq : 0u) + k * st;
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const float *row = T + (size_t)(r * L) * st;
@@ -1277,15 +1322,15 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                         acc[r] = stage_f[i * W + q];
                 }
             } else if constexpr (PACKED) {
-                /* The staged row with its entries j >= y set to +0; every
-                 * table entry j >= y is +0 too (load_pass), so each such
-                 * product is +-0 and adding it leaves the sum unchanged (a
-                 * sum that starts at +0 is never -0): the unmasked sums
-                 * equal the sums over j < y bit for bit. */
+                /* The staged row holds +0 from y on (staging), and so does
+                 * every table entry j >= y (load_pass): each such product
+                 * is +0 and adding it leaves the sum unchanged (a sum that
+                 * starts at +0 is never -0), so the unmasked sums equal the
+                 * sums over j < y bit for bit. */
                 float b[S];
 #pragma unroll
                 for (int j = 0; j < S; ++j)
-                    b[j] = (unsigned)j < y ? stage_f[j * W + q] : 0.0f;
+                    b[j] = stage_f[j * W + q];
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
                     unsigned i = k + r * L;
@@ -1323,15 +1368,19 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
             /* the row sums in row order, from the lanes that hold them */
             float sum = 0.0f;
             if constexpr (PACKED && L == 2) {
-                /* rows 2r and 2r + 1 sit in acc[r] of the pair's two
-                 * lanes: one swap of neighbouring lanes (DPP) per r.  Rows
-                 * past x hold +0, which leaves the sum unchanged. */
+                /* rows 2r and 2r + 1 sit in acc[r] of the pair's low and
+                 * high lane: the low lane adds them in row order, taking
+                 * each odd row from its neighbour by one DPP swap, and the
+                 * high lane takes the low lane's sum.  Rows past x hold +0,
+                 * which leaves the sum unchanged. */
 #pragma unroll
                 for (int r = 0; r < NR; ++r) {
                     const float o = pair_swap(acc[r]);
-                    sum = sum + (k ? o : acc[r]);
-                    sum = sum + (k ? acc[r] : o);
+                    sum = sum + acc[r];
+                    sum = sum + o;
                 }
+                const float lo = pair_swap(sum);
+                sum = k ? lo : sum;
             } else {
 #pragma unroll
                 for (int i = 0; i < S; ++i) {
@@ -1372,6 +1421,8 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
             break; /* the chunk's slots are all done */
         float tv[NR][S];
         load_pass(p, tv);
+        if constexpr (PACKED)
+            __builtin_amdgcn_sched_barrier(0); /* every table load issued before the first product */
         run_pass(p, tv);
     }
     wave_sync();
