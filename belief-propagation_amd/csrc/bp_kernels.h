@@ -289,6 +289,38 @@ __device__ __forceinline__ void send(const float *__restrict__ Jrow, const float
         m[i] = m[i] / sum;
 }
 
+/* send() for a table whose entries past y are +0 in every row i < x (the
+ * zero padding of mixed-state rows read by load_mixed_rows): the source row
+ * is zeroed from y on once, every product is then taken, and rows i >= x
+ * (whatever lies there) are dropped after their sums.  Each extra product
+ * is +0, and adding +0 leaves a sum unchanged (a sum that starts at +0 is
+ * never -0), so the results are send()'s bit for bit with one select per
+ * entry of the row instead of one per product. */
+template <int S>
+__device__ __forceinline__ void send_padded(const float *__restrict__ Jrow, const float *b, unsigned x, unsigned y,
+                                            float *m)
+{
+    float bb[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j)
+        bb[j] = (unsigned)j < y ? b[j] : 0.0f;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int j = 0; j < S; ++j)
+            acc = acc + Jrow[i * S + j] * bb[j];
+        m[i] = (unsigned)i < x ? acc : 0.0f;
+        sum = sum + m[i];
+    }
+    if (sum <= 0.0f)
+        sum = 1.0f;
+#pragma unroll
+    for (int i = 0; i < S; ++i)
+        m[i] = m[i] / sum;
+}
+
 /* Row i of a pooled table (rows of y entries padded to a multiple of four)
  * into r[0..S): aligned four-float loads, zero beyond y. */
 template <int S>
@@ -1084,7 +1116,9 @@ __device__ __forceinline__ void chunk_narrow(const LoopyArgs &a, unsigned nb, un
                 unsigned x = xy[r] & 15u, y = xy[r] >> 4;
                 if constexpr (POOL)
                     pool_table<S>(praw[r], (tix[r] >> 31) != 0, J[r]);
-                if constexpr (HOLD)
+                if constexpr (OFFT)
+                    send_padded<S>(J[r], bsrc[r], x, y, m);
+                else if constexpr (HOLD)
                     send<S, MIXED>(J[r], bsrc[r], x, y, m);
                 else
                     send_from_memory<S, MIXED, PACKED, POOLED>(a, s, xy[r], bsrc[r], m);

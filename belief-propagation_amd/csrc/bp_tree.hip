@@ -129,7 +129,7 @@ __device__ __forceinline__ void tree_entry(const TreeArgs &a, unsigned entry, in
         if constexpr (MIXED) {
             float Jd[S * S];
             load_mixed_rows<S>(a.J + a.slot_tab[s], Jd);
-            send<S, MIXED>(Jd, buf, x, y, m);
+            send_padded<S>(Jd, buf, x, y, m);
         } else {
             send<S, MIXED>(a.J + (size_t)s * S * S, buf, x, y, m);
         }
