@@ -879,7 +879,9 @@ __device__ __forceinline__ float pair_swap(float v)
  * loaded them, one lane per node).  The normalising sum keeps the order of
  * the one-lane form: the low lane adds its entries to 0, the high lane
  * continues from that partial, and the total goes back to the low lane.
- * Entries past the node's count n are never summed or stored. */
+ * As in node_update, each message's NaN entries are replaced by 1 once for
+ * both products.  Entries past the node's count n are never summed or
+ * stored. */
 template <int S, int MODE>
 __device__ __forceinline__ void node_update_pair(const LoopyArgs &a, unsigned nrow, unsigned q0, unsigned q1, unsigned n,
                                                  bool ok, const float *lds, unsigned lane)
