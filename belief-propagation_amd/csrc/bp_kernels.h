@@ -265,7 +265,11 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-/* m = N(J . b): m[i] = sum_j J[i][j] * b[j] for i < x, j < y (graph.c:896-909) */
+/* m = N(J . b): m[i] = sum_j J[i][j] * b[j] for i < x, j < y (graph.c:896-909).
+ * Entries past x are +0, the zero extension a node's products read past
+ * the message's own entries, also when the sum is NaN (an infinite table
+ * entry): 0 / NaN would be NaN, which the products skip instead of
+ * multiplying by 0. */
 template <int S, bool MIXED>
 __device__ __forceinline__ void send(const float *__restrict__ Jrow, const float *b, unsigned x, unsigned y, float *m)
 {
@@ -286,7 +290,7 @@ __device__ __forceinline__ void send(const float *__restrict__ Jrow, const float
         sum = 1.0f;
 #pragma unroll
     for (int i = 0; i < S; ++i)
-        m[i] = m[i] / sum;
+        m[i] = (!MIXED || (unsigned)i < x) ? m[i] / sum : 0.0f;
 }
 
 /* send() for a table whose entries past y are +0 in every row i < x (the
@@ -318,7 +322,7 @@ __device__ __forceinline__ void send_padded(const float *__restrict__ Jrow, cons
         sum = 1.0f;
 #pragma unroll
     for (int i = 0; i < S; ++i)
-        m[i] = m[i] / sum;
+        m[i] = (unsigned)i < x ? m[i] / sum : 0.0f;
 }
 
 /* Row i of a pooled table (rows of y entries padded to a multiple of four)
@@ -478,7 +482,7 @@ __device__ __forceinline__ void send_from_memory(const LoopyArgs &a, unsigned s,
         sum = 1.0f;
 #pragma unroll
     for (int i = 0; i < S; ++i)
-        m[i] = m[i] / sum;
+        m[i] = (!MIXED || (unsigned)i < x) ? m[i] / sum : 0.0f;
 }
 
 /* ---- compact rows (PACKED: the batch layout) ------------------------------
@@ -1429,7 +1433,7 @@ __device__ __forceinline__ void chunk_rows(const LoopyArgs &a, unsigned nb, unsi
                 sum = 1.0f;
 #pragma unroll
             for (int r = 0; r < NR; ++r)
-                m[r] = acc[r] / sum;
+                m[r] = (!MIXED || k + (unsigned)(r * L) < x) ? acc[r] / sum : 0.0f;
         }
         if (act) {
 #pragma unroll
