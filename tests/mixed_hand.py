@@ -155,6 +155,47 @@ def chain_expected():
     return {"init": [P0, P1, P2, P3], "msgs": [C0, C1, C2, C3], "states": [s_a, s_b, s_c]}
 
 
+NAN = float("nan")
+
+
+def chain_graph_inf():
+    """chain_graph() with J_ab[0][0] = +inf (a network file can hold an
+    infinite weight): a's message to b then has an infinite sum."""
+    g = chain_graph()
+    G = g.contents
+    E = G.current_num_edges
+    Jv = B.view_f32(G.edges_joint_probabilities, E * MAX_STATES * MAX_STATES).reshape(E, MAX_STATES, MAX_STATES)
+    Jv[0, 0, 0] = np.inf
+    return g
+
+
+def chain_inf_expected():
+    """The chain with J_ab[0][0] = +inf, by the same rules and two more:
+    N() of a vector with an infinite sum gives NaN for the infinite entry and
+    0 for a finite one (IEEE division), and a product skips a NaN factor
+    (graph.c:450-459).  The zero extension does not look at a message's
+    values: the entry past a NaN message's x still reads 0."""
+    one3 = [F(1)] * 3
+    P0 = [NAN, F(0)]                     # J_ab . (S_a, 0) = (inf, 7/8)
+    P1 = N(Jb(J_ba, one3[:2]))
+    P2 = N(Jb(J_bc, one3))
+    P3 = N(Jb(J_cb, S_c))
+    buf_a = times(S_a, P1[:2])
+    buf_c = times(S_c, P2)
+    # b's buffer: P0[0] is NaN (skipped), P0[1] = 0, entry 2 reads 0
+    buf_b = [P3[0], F(0), F(0)]
+    C0 = [NAN, F(0)]                     # J_ab . (buf_a, 0): row 0 infinite again
+    C1 = N(Jb(J_ba, buf_b[:2]))
+    C2 = N(Jb(J_bc, buf_b))
+    C3 = N(Jb(J_cb, buf_c))
+    assert C1 == [F(1, 7), F(2, 7), F(4, 7)] and C2 == [F(4, 13), F(1, 13), F(8, 13)]
+    s_a = N(times(S_a, C1[:2]))
+    s_b = N([C3[0], F(0), F(0)])         # C0[0] skipped, C0[1] = 0, entry 2 reads 0
+    s_c = N(times(S_c, C2))
+    assert s_b == [F(1), F(0), F(0)] and s_a == [F(1, 7), F(6, 7)]
+    return {"init": [P0, P1, P2, P3], "msgs": [C0, C1, C2, C3], "states": [s_a, s_b, s_c]}
+
+
 def star_graph():
     """b(3) with leaves a(2), c(3), d(2): nodes 0=a, 1=b, 2=c, 3=d; edges
     e0 a->b, e1 b->a, e2 c->b, e3 b->c, e4 d->b, e5 b->d."""
@@ -196,7 +237,7 @@ def rows(flat_rows: np.ndarray, counts) -> list:
 
 def mismatches(got: list, want: list, what: str) -> list[str]:
     """Entries of got (float rows) that differ from the exact rows want by
-    more than RTOL relative (an exact 0 must be 0)."""
+    more than RTOL relative (an exact 0 must be 0, an expected NaN NaN)."""
     out = []
     for i, (gr, wr) in enumerate(zip(got, want)):
         if len(gr) != len(wr):
@@ -204,6 +245,10 @@ def mismatches(got: list, want: list, what: str) -> list[str]:
             continue
         for k, (a, b) in enumerate(zip(gr, wr)):
             b = float(b)
+            if b != b:
+                if a == a:
+                    out.append(f"{what}[{i}][{k}] = {a!r}, expected NaN")
+                continue
             if not (abs(a - b) <= RTOL * abs(b)):
                 out.append(f"{what}[{i}][{k}] = {a!r}, expected {b!r}")
     return out

@@ -282,6 +282,22 @@ def test_mixed_states_hand_worked_loopy(lib, orc):
         lib.graph_destroy(g)
 
 
+def test_mixed_states_hand_worked_nan_message(lib, orc):
+    """Zero extension past a NaN message: with an infinite entry in a's
+    table to b, a's message is (NaN, 0); b's products skip the NaN entry,
+    multiply by the 0 and by the 0 read past the message's two entries
+    (graph.c:450-459, 985-1024), against values worked out by hand in
+    tests/mixed_hand.py; serial and OpenMP node loops."""
+    exp = MH.chain_inf_expected()
+    for threads in (1, 2):
+        g = MH.chain_graph_inf()
+        orc.init_previous_edge(g)
+        assert MH.check_chain_init(g, exp) == []
+        orc.loopy_iteration(g, threads=threads)
+        assert MH.check_chain_iteration(g, exp) == []
+        lib.graph_destroy(g)
+
+
 def test_mixed_states_hand_worked_tree(lib, orc):
     """Zero-extension in tree BP (graph.c:461-538, 540-596): a 3-state node
     combining two 2-entry messages and sending on a 3x3 edge, against exact
