@@ -123,6 +123,18 @@ def same_bits(a: np.ndarray, b: np.ndarray) -> bool:
     return a.shape == b.shape and a.tobytes() == np.ascontiguousarray(b).tobytes()
 
 
+def nan_equal_bits(a: np.ndarray, b: np.ndarray) -> bool:
+    """same_bits, except that any NaN equals any NaN: x86 produces the
+    negative default NaN for inf / inf or inf - inf, the GPU the positive
+    one.  NaN positions must agree."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    if a.shape != b.shape:
+        return False
+    na, nb = np.isnan(a), np.isnan(b)
+    return bool(np.array_equal(na, nb)) and a[~na].tobytes() == b[~nb].tobytes()
+
+
 def bits_report(a: np.ndarray, b: np.ndarray) -> str:
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import bp_ctypes as B
-from conftest import (SMALL_LOOPY_CASES, bits_report, golden, graph_for_case, messages_of, same_bits,
+from conftest import (SMALL_LOOPY_CASES, bits_report, golden, graph_for_case, messages_of, nan_equal_bits, same_bits,
                       states_of)
 
 pytestmark = pytest.mark.gpu
@@ -129,18 +129,24 @@ def test_until_bitwise(lib, case):
         lib.graph_destroy(g)
 
 
+@pytest.mark.parametrize("case", ["finite", "inf"])
 @pytest.mark.parametrize("path", ["handle", "dropin"])
-def test_mixed_states_hand_worked(lib, orc, path):
+def test_mixed_states_hand_worked(lib, orc, path, case):
     """Zero-extension on the GPU against exact hand-worked values
     (tests/mixed_hand.py: a 2-state node sending on a 2x3 edge, a 3-state
     node combining a 2-entry message, the belief update over it; every
-    padded slot filled with 7.0), and bit for bit against the oracle."""
+    padded slot filled with 7.0), and bit for bit against the oracle.  The
+    inf case puts an infinite entry in that edge's table, so the 2-entry
+    message is NaN in one entry: the zero extension past it must still read
+    0 (NaN positions compared, not NaN bit patterns)."""
     import mixed_hand as MH
 
-    print(f"mixed chain, {path}", flush=True)
-    exp = MH.chain_expected()
-    g = MH.chain_graph()
-    h = MH.chain_graph()
+    print(f"mixed chain, {path}, {case}", flush=True)
+    exp = MH.chain_expected() if case == "finite" else MH.chain_inf_expected()
+    build = MH.chain_graph if case == "finite" else MH.chain_graph_inf
+    eq = same_bits if case == "finite" else nan_equal_bits
+    g = build()
+    h = build()
     if path == "handle":
         dev = B.Device(g)
         dev.init()
@@ -149,7 +155,7 @@ def test_mixed_states_hand_worked(lib, orc, path):
         lib.init_previous_edge(g)
     assert MH.check_chain_init(g, exp) == []
     orc.init_previous_edge(h)
-    assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
+    assert eq(B.messages(g, "prev"), B.messages(h, "prev"))
     if path == "handle":
         dev.iterate(1)
         dev.download(g)
@@ -158,9 +164,9 @@ def test_mixed_states_hand_worked(lib, orc, path):
         lib.loopy_propagate_one_iteration(g)
     assert MH.check_chain_iteration(g, exp) == []
     orc.loopy_iteration(h)
-    assert same_bits(B.states(g), B.states(h))
-    assert same_bits(B.messages(g, "prev"), B.messages(h, "prev"))
-    assert same_bits(B.messages(g, "cur"), B.messages(h, "cur"))
+    assert eq(B.states(g), B.states(h))
+    assert eq(B.messages(g, "prev"), B.messages(h, "prev"))
+    assert eq(B.messages(g, "cur"), B.messages(h, "cur"))
     lib.graph_destroy(g)
     lib.graph_destroy(h)
 

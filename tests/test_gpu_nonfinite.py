@@ -17,17 +17,9 @@ import numpy as np
 import pytest
 
 import bp_ctypes as B
+from conftest import nan_equal_bits
 
 pytestmark = pytest.mark.gpu
-
-
-def nan_equal_bits(a: np.ndarray, b: np.ndarray) -> bool:
-    a = np.ascontiguousarray(a, dtype=np.float32)
-    b = np.ascontiguousarray(b, dtype=np.float32)
-    if a.shape != b.shape:
-        return False
-    na, nb = np.isnan(a), np.isnan(b)
-    return bool(np.array_equal(na, nb)) and a[~na].tobytes() == b[~nb].tobytes()
 
 
 def mismatch_report(a: np.ndarray, b: np.ndarray, limit: int = 6) -> str:
