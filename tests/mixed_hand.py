@@ -231,6 +231,39 @@ def star_expected():
             "levels": [0, 2, 3, 1]}
 
 
+def star_graph_inf():
+    """star_graph() with J_ab[0][0] = +inf: a's message to b is (NaN, 0)."""
+    g = star_graph()
+    G = g.contents
+    E = G.current_num_edges
+    Jv = B.view_f32(G.edges_joint_probabilities, E * MAX_STATES * MAX_STATES).reshape(E, MAX_STATES, MAX_STATES)
+    Jv[0, 0, 0] = np.inf
+    return g
+
+
+def star_inf_expected():
+    """Tree BP on the star with J_ab[0][0] = +inf, by the rules of
+    chain_inf_expected: b's buffer skips E0's NaN entry, multiplies by its 0
+    and by the 0 read past its two entries."""
+    E0 = [NAN, F(0)]                     # J_ab . (S_a, 0) = (inf, 7/8)
+    E2 = N(Jb(J_cb, S_c))
+    E4 = N(Jb(J_db, S_d + [0]))
+    v = E2[0] * E4[0]
+    buf = [v, F(0), F(0)]
+    E1 = N(Jb(J_ba, buf[:2]))
+    E3 = N(Jb(J_bc, buf))
+    E5 = N(Jb(J_bd, buf[:2]))
+    assert E1 == [F(1, 7), F(2, 7), F(4, 7)] and E3 == [F(4, 13), F(1, 13), F(8, 13)]
+    assert E5 == [F(1, 2), F(1, 4), F(1, 4)]
+    s_a = N(E1[:2])
+    s_b = N(buf)
+    s_c = N(E3)
+    s_d = N(E5[:2])
+    assert s_b == [F(1), F(0), F(0)] and s_d == [F(2, 3), F(1, 3)]
+    return {"msgs": [E0, E1, E2, E3, E4, E5], "states": [s_a, s_b, s_c, s_d], "num_levels": 2,
+            "levels": [0, 2, 3, 1]}
+
+
 def rows(flat_rows: np.ndarray, counts) -> list:
     return [[float(x) for x in flat_rows[i, :n]] for i, n in enumerate(counts)]
 

@@ -312,6 +312,17 @@ def test_mixed_states_hand_worked_tree(lib, orc):
     lib.graph_destroy(g)
 
 
+def test_mixed_states_hand_worked_tree_nan_message(lib, orc):
+    """The star with an infinite entry in a's table to b: tree BP skips the
+    NaN entry of a's message at b and still reads 0 past its two entries
+    (graph.c:450-459, 472-521), against tests/mixed_hand.py."""
+    exp = MH.star_inf_expected()
+    g = MH.star_graph_inf()
+    orc.tree_run(g)
+    assert MH.check_star(g, exp) == []
+    lib.graph_destroy(g)
+
+
 @pytest.mark.parametrize("plant", ["pad_one", "stale", "norm_stride"])
 def test_mixed_hand_worked_rejects_planted_mistakes(lib, orc, plant):
     """Each plausible mistake in the mixed-state branch, planted in a copy of
